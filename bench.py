@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: random brickwork circuit, 30 qubits, depth 20, complex128.
+
+A step is one execution of the whole circuit (600 single-qubit rotations +
+290 CNOTs) on a 2^30-amplitude state resident in HBM.  `value` is the gate
+throughput in GB/s: the algorithmic bytes of every gate (RX/RY/RZ read+write
+all amplitudes, 2*c*N; CNOT reads+writes the control=1 half, c*N) summed over
+the circuit and divided by the wall time of the timed steps.
+
+The line also carries the roofline of the dominant kernel (the single-qubit
+pair kernel, timed live with HIP events on the library stream) and a CPU
+baseline: the oracle's single-threaded gate loops (the reference runs its gate
+loops on one thread) on a bounded sample of the same 30-qubit workload.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--mode", choices=["per_gate", "fused"], default="per_gate")
+    ap.add_argument("--cpu-qubits", type=int, default=30)
+    ap.add_argument("--cpu-gates", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(descs, n, c):
+    from logosq_amd import abi
+    N = 1 << n
+    total = 0
+    for g in descs:
+        if g.kind in (abi.RX, abi.RY, abi.RZ, abi.U3, abi.H, abi.MATRIX1, abi.RZZ):
+            total += 2 * c * N
+        elif g.kind in (abi.CNOT, abi.SWAP, abi.CRX, abi.CRY, abi.CRZ, abi.PHASE, abi.Z, abi.S, abi.T, abi.X,
+                        abi.Y):
+            total += c * N
+        elif g.kind in (abi.CZ, abi.CPHASE, abi.CP, abi.TOFFOLI):
+            total += c * N // 2
+        else:
+            total += 2 * c * N
+    return total
+
+
+def time_pair_kernel(lq, st, n, c):
+    """Average duration of the single-qubit pair kernel (RX on every target),
+    measured with HIP events on the library stream."""
+    from logosq_amd import abi
+    lib = lq.load()
+    gates = [abi.make_gate(abi.RX, [q], [0.1 + q]) for q in range(n)]
+    arr = abi.gate_array(gates)
+    reps = 2
+    lq._lib.call("lq_synchronize")
+    lq._lib.call("lq_event_record", 0)
+    for _ in range(reps):
+        lq._lib.call("lq_execute", st.handle, arr, len(gates), abi.EXEC_PER_GATE)
+    lq._lib.call("lq_event_record", 1)
+    ms = ctypes.c_float()
+    lq._lib.call("lq_event_elapsed_ms", 0, 1, ctypes.byref(ms))
+    launches = reps * len(gates)
+    avg_s = ms.value / 1e3 / launches
+    bytes_per_launch = 2 * c * (1 << n)
+    return avg_s, bytes_per_launch
+
+
+def cpu_baseline(nq, ngates, seed, depth):
+    """The oracle's restated gate loops, single thread, on the first gates of
+    the same circuit at nq qubits."""
+    import numpy as np
+    import oracle
+    from logosq_amd import algorithms, abi
+    lib = oracle.lib()
+    descs = algorithms.brickwork_circuit(nq, depth, seed).descriptors()
+    layer0 = descs[:nq] + descs[nq:nq + nq // 2]
+    sample = [layer0[0], layer0[nq // 2], layer0[nq]][:ngates] if ngates <= 3 else layer0[:ngates]
+    v = np.zeros(1 << nq, dtype=np.complex128)
+    v[:] = 0  # touch every page before timing
+    v[0] = 1
+    p = v.ctypes.data_as(ctypes.c_void_p)
+    total_bytes = algorithmic_bytes(sample, nq, 16)
+    t0 = time.perf_counter()
+    for g in sample:
+        lib.or_apply_gate(p, nq, ctypes.byref(g))
+    dt = time.perf_counter() - t0
+    names = {abi.RX: "RX", abi.RY: "RY", abi.RZ: "RZ", abi.CNOT: "CNOT"}
+    return {"value": total_bytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "%d gates of layer 0 (%s) at %d qubits c128, oracle gate loops single-threaded as the "
+                      "reference's, %.1f s" % (len(sample), ",".join(names.get(g.kind, str(g.kind)) for g in sample),
+                                              nq, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d requested)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import logosq_amd as lq
+    from logosq_amd import abi
+    lq.load()
+    dtype = lq.C128 if a.dtype == "c128" else lq.C64
+    c = 16 if dtype == lq.C128 else 8
+    mode = abi.EXEC_PER_GATE if a.mode == "per_gate" else abi.EXEC_FUSED
+    circ = lq.algorithms.brickwork_circuit(a.qubits, a.depth, a.seed)
+    descs = circ.descriptors()
+    arr = abi.gate_array(descs)
+    st = lq.State(a.qubits, dtype)
+    print("bench: %d qubits %s depth %d, %d gates, mode %s" % (a.qubits, a.dtype, a.depth, len(descs), a.mode),
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        lq._lib.call("lq_state_set_basis", st.handle, 0)
+        lq._lib.call("lq_execute", st.handle, arr, len(descs), mode)
+    lq._lib.call("lq_synchronize")
+    times = []
+    for _ in range(a.steps):
+        lq._lib.call("lq_state_set_basis", st.handle, 0)
+        lq._lib.call("lq_synchronize")
+        t0 = time.perf_counter()
+        lq._lib.call("lq_execute", st.handle, arr, len(descs), mode)
+        lq._lib.call("lq_synchronize")
+        times.append(time.perf_counter() - t0)
+        print("bench: step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    norm2 = st.norm_sqr()
+    step_s = sum(times) / len(times)
+    bytes_circ = algorithmic_bytes(descs, a.qubits, c)
+    value = bytes_circ / step_s / 1e9
+    avg_s, bpl = time_pair_kernel(lq, st, a.qubits, c)
+    achieved = bpl / avg_s / 1e9
+    cpu = None
+    if not a.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(a.cpu_qubits, a.cpu_gates, a.seed, a.depth)
+    out = {
+        "metric": "gate HBM GB/s at 30q c128; VQE time-to-dE<1e-4, warm-start+batched vs plain",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.dtype,
+        "data": "synthetic",
+        "config": {"workload": "random brickwork circuit, %d qubits, depth %d, %s, seed %d, %s execution"
+                               % (a.qubits, a.depth, a.dtype, a.seed, a.mode.replace("_", "-")),
+                   "qubits": a.qubits, "depth": a.depth, "gates": len(descs),
+                   "algorithmic_bytes_per_step": bytes_circ, "final_norm_sqr": norm2},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pair (RX on each of the %d targets), %d B per launch, avg %.3f ms"
+                               % (a.qubits, bpl, avg_s * 1e3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

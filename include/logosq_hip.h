@@ -1,0 +1,241 @@
+/*
+ * logosq_hip.h -- C ABI of the MI355X (gfx950) dense state-vector backend.
+ *
+ * This is the drop-in boundary under LogosQ's State / Gate / Circuit /
+ * Observable / GradientMethod API.  The reference crate (logosq 0.2.5) has no
+ * FFI of its own: its boundary is the Rust trait API, and every entry point
+ * below names the reference item it replaces (file:line in the reference
+ * tree).  A Rust `extern "C"` block that binds these symbols is the only
+ * glue a LogosQ maintainer needs.
+ *
+ * Conventions (identical to the reference):
+ *   - amplitudes are complex, interleaved (re, im); complex128 = two f64
+ *     (`num_complex::Complex64`, src/prelude.rs:180), complex64 = two f32;
+ *   - qubit q is bit (n-1-q) of the basis index, so qubit 0 is the MSB
+ *     (src/gates/utils.rs:5-7);
+ *   - matrices are row-major, interleaved complex doubles.
+ *
+ * Errors: every function returns an lq_status and never aborts; the text of
+ * the last error on the calling thread is available from lq_last_error().
+ * Threading: calls on distinct state handles are thread safe; calls on one
+ * handle must be serialised by the caller (the reference takes &mut State).
+ */
+#ifndef LOGOSQ_HIP_H
+#define LOGOSQ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LQ_ABI_VERSION 1
+
+/* Status codes mirror LogosQError (src/error.rs:9-62). */
+typedef enum lq_status {
+    LQ_OK = 0,
+    LQ_ERR_INVALID_QUBIT_INDEX = 1,      /* InvalidQubitIndex      src/error.rs:11-12 */
+    LQ_ERR_INVALID_STATE_DIMENSION = 2,  /* InvalidStateDimension  src/error.rs:15-16 */
+    LQ_ERR_DIMENSION_MISMATCH = 3,       /* DimensionMismatch      src/error.rs:19-20 */
+    LQ_ERR_INVALID_GATE_MATRIX = 4,      /* InvalidGateMatrix      src/error.rs:23-27 */
+    LQ_ERR_PARAMETER_COUNT_MISMATCH = 5, /* ParameterCountMismatch src/error.rs:30-31 */
+    LQ_ERR_CIRCUIT_QUBIT_MISMATCH = 6,   /* CircuitQubitMismatch   src/error.rs:34-40 */
+    LQ_ERR_INVALID_ARGUMENT = 20,
+    LQ_ERR_OUT_OF_MEMORY = 21,
+    LQ_ERR_DEVICE = 22,
+    LQ_ERR_UNSUPPORTED = 23
+} lq_status;
+
+typedef enum lq_dtype {
+    LQ_C128 = 0, /* complex128: the reference's Complex64 */
+    LQ_C64 = 1   /* complex64 (two f32); reductions still accumulate in f64 */
+} lq_dtype;
+
+/* Gate kinds.  Each names the reference struct whose `Gate::apply` loop it
+ * replaces. */
+typedef enum lq_gate_kind {
+    LQ_GATE_X = 1,        /* XGate        src/gates/single_qubit.rs:14-31 */
+    LQ_GATE_Y = 2,        /* YGate        src/gates/single_qubit.rs:39-59 (applies -Y) */
+    LQ_GATE_Z = 3,        /* ZGate        src/gates/single_qubit.rs:67-82 */
+    LQ_GATE_H = 4,        /* HGate        src/gates/single_qubit.rs:90-113 */
+    LQ_GATE_S = 5,        /* SGate        src/gates/single_qubit.rs:121-137 */
+    LQ_GATE_T = 6,        /* TGate        src/gates/single_qubit.rs:146-161 */
+    LQ_GATE_RX = 10,      /* RXGate       src/gates/parameterized.rs:48-78 */
+    LQ_GATE_RY = 11,      /* RYGate       src/gates/parameterized.rs:101-131 */
+    LQ_GATE_RZ = 12,      /* RZGate       src/gates/parameterized.rs:154-176 */
+    LQ_GATE_PHASE = 13,   /* PhaseGate    src/gates/parameterized.rs:199-218 */
+    LQ_GATE_U3 = 14,      /* U3Gate       src/gates/parameterized.rs:249-282 */
+    LQ_GATE_CNOT = 20,    /* CNOTGate     src/gates/two_qubit.rs:12-31 */
+    LQ_GATE_SWAP = 21,    /* SWAPGate     src/gates/two_qubit.rs:40-62 */
+    LQ_GATE_CZ = 22,      /* CZGate       src/gates/two_qubit.rs:71-90 */
+    LQ_GATE_CPHASE = 23,  /* CPhaseGate   src/gates/two_qubit.rs:100-136 */
+    LQ_GATE_CRX = 30,     /* CRXGate      src/gates/parameterized.rs:311-343 */
+    LQ_GATE_CRY = 31,     /* CRYGate      src/gates/parameterized.rs:368-399 */
+    LQ_GATE_CRZ = 32,     /* CRZGate      src/gates/parameterized.rs:424-451 */
+    LQ_GATE_CP = 33,      /* CPhaseParamGate src/gates/parameterized.rs:476-499 */
+    LQ_GATE_RXX = 40,     /* RXXGate      src/gates/parameterized.rs:528-578 */
+    LQ_GATE_RYY = 41,     /* RYYGate      src/gates/parameterized.rs:604-653 */
+    LQ_GATE_RZZ = 42,     /* RZZGate      src/gates/parameterized.rs:679-709 */
+    LQ_GATE_TOFFOLI = 50, /* ToffoliGate  src/gates/three_qubit.rs:13-35 */
+    /* Dense builder gates: Circuit::x/y/z/h/s/t/rx_matrix/... build a full
+     * 2^n x 2^n MatrixGate (src/circuits/single_qubit.rs:12-63) whose apply is
+     * `dot` followed by State::normalize (src/gates/mod.rs:77-93).  The
+     * backend applies the 2x2 block to qubits[0] and renormalises, which is
+     * the same operator without the O(4^n) matrix. */
+    LQ_GATE_MATRIX1 = 60,
+    /* Circuit::add_two_qubit_gate (src/circuits/two_qubits.rs:11-82): 4x4
+     * block on the sorted pair (q1 < q2), sub-index (bit(q1) << 1) | bit(q2),
+     * then renormalise. */
+    LQ_GATE_MATRIX2 = 61
+} lq_gate_kind;
+
+/* Flag bits of lq_gate.flags. */
+#define LQ_GATE_FLAG_RENORMALIZE 1 /* renormalise after the gate (MatrixGate semantics) */
+
+/* One gate.  `theta[k]` is the k-th angle (theta, phi, lambda for U3).  When
+ * `param[k] >= 0` the angle is parameter slot param[k] of an ansatz template
+ * (ParameterizedCircuit, src/optimization/ansatz.rs:298-335) and theta[k] is
+ * ignored by template-consuming entry points. */
+typedef struct lq_gate {
+    int32_t kind;
+    int32_t qubits[3];
+    double theta[3];
+    int32_t param[3];
+    int32_t flags;
+    double matrix[32]; /* MATRIX1: 2x2 (8 doubles); MATRIX2: 4x4 (32 doubles) */
+} lq_gate;
+
+/* Pauli codes of Pauli (src/optimization/observable.rs:22-27). */
+#define LQ_PAULI_I 0
+#define LQ_PAULI_X 1
+#define LQ_PAULI_Y 2
+#define LQ_PAULI_Z 3
+
+/* Y-phase convention of PauliTerm::expectation.  The reference takes the Y
+ * phase from the row bit (src/optimization/observable.rs:125-133), which
+ * returns (-1)^{#Y} <P>; LQ_PAULI_REFERENCE reproduces it bit for bit,
+ * LQ_PAULI_PHYSICAL returns <P>. */
+#define LQ_PAULI_REFERENCE 0
+#define LQ_PAULI_PHYSICAL 1
+
+/* Execution modes of lq_execute. */
+#define LQ_EXEC_PER_GATE 0 /* one kernel per gate, in order (Gate::apply)          */
+#define LQ_EXEC_FUSED 1    /* LDS tile passes that apply runs of gates per pass      */
+
+typedef struct lq_state lq_state; /* opaque device-resident State */
+
+/* ---- library ---------------------------------------------------------- */
+const char* lq_last_error(void);
+int lq_abi_version(void);
+/* Device name, CU count, LDS bytes per block, global memory bytes. */
+int lq_device_info(char* name, size_t name_len, int* cu_count, size_t* lds_bytes, size_t* global_bytes);
+/* Block until all work queued by the library on its stream is done. */
+int lq_synchronize(void);
+
+/* Timing on the library stream (no counterpart in the reference): record a
+ * HIP event in slot [0, 64) and read the time between two recorded slots. */
+int lq_event_record(int slot);
+int lq_event_elapsed_ms(int slot_start, int slot_end, float* ms);
+
+/* ---- State (src/states/mod.rs:38-392) ---------------------------------- */
+/* State::zero_state (mod.rs:136-141): |0...0>.  n in [1, 34]. */
+int lq_state_create(int num_qubits, int dtype, lq_state** out);
+int lq_state_destroy(lq_state* st);
+int lq_state_num_qubits(const lq_state* st, int* num_qubits);
+int lq_state_dtype(const lq_state* st, int* dtype);
+/* zero_state / one_state (mod.rs:136-149): basis state |index>. */
+int lq_state_set_basis(lq_state* st, uint64_t index);
+/* plus_state (mod.rs:152-157): every amplitude 1/sqrt(2^n). */
+int lq_state_set_plus(lq_state* st);
+/* State::new / new_unchecked (mod.rs:55-86) when normalize != 0; a raw
+ * vector_mut() write (mod.rs:131-133) when normalize == 0.  `amps` holds len
+ * interleaved complex doubles; len must equal 2^n (InvalidStateDimension). */
+int lq_state_upload(lq_state* st, const double* amps, uint64_t len, int normalize);
+/* State::vector (mod.rs:122-124): copy the amplitudes to the host. */
+int lq_state_download(const lq_state* st, double* amps, uint64_t len);
+/* State: Clone (mod.rs:37).  dst and src must have the same n and dtype. */
+int lq_state_copy(lq_state* dst, const lq_state* src);
+/* State::normalize (mod.rs:91-114): divide by the norm when it exceeds 1e-10.
+ * norm_out (nullable) receives the norm before scaling. */
+int lq_state_normalize(lq_state* st, double* norm_out);
+/* sum |a_i|^2 in f64. */
+int lq_state_norm_sqr(const lq_state* st, double* out);
+/* inner_product / inner_product_parallel (mod.rs:350-391): <a|b>. */
+int lq_state_inner_product(const lq_state* a, const lq_state* b, double out[2]);
+/* probabilities (mod.rs:277-279): len = 2^n doubles. */
+int lq_state_probabilities(const lq_state* st, double* out, uint64_t len);
+
+/* ---- Gates (Gate::apply, src/gates/mod.rs:26-35) ------------------------ */
+/* Apply one gate with the kernel for its kind (the per-gate drop-in). */
+int lq_apply_gate(lq_state* st, const lq_gate* gate);
+/* QuantumStateBackend::apply_single_qubit_matrix without the renormalise
+ * (src/states/backend.rs:23-24): 2x2 matrix m on `qubit`. */
+int lq_apply_1q(lq_state* st, int qubit, const double m[8]);
+/* 2x2 matrix on `target` in the subspace where every control qubit is 1. */
+int lq_apply_ctrl_1q(lq_state* st, const int* controls, int num_controls, int target, const double m[8]);
+/* QuantumStateBackend::apply_two_qubit_matrix without the renormalise
+ * (src/states/mod.rs:441-456): 4x4 matrix, sub-index (bit(q_a) << 1) | bit(q_b). */
+int lq_apply_2q(lq_state* st, int q_a, int q_b, const double m[32]);
+/* MatrixGate::apply (src/gates/mod.rs:77-93): dense dim x dim matvec, then
+ * State::normalize.  dim must equal 2^n (else DimensionMismatch). */
+int lq_apply_dense(lq_state* st, const double* matrix, uint64_t dim);
+
+/* ---- Circuits (src/circuits/mod.rs:352-375) ----------------------------- */
+/* Circuit::execute: apply gates[0..count) in order.  mode is LQ_EXEC_*.
+ * Angles come from theta[] (param[] is ignored). */
+int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode);
+
+/* ---- Observables (src/optimization/observable.rs) ----------------------- */
+/* PauliObservable::expectation (observable.rs:228-231) =
+ * sum_t coeff[t] * PauliTerm::expectation (observable.rs:95-147).
+ * paulis is num_terms x n codes (LQ_PAULI_*), row t for term t, column q for
+ * qubit q.  convention is LQ_PAULI_REFERENCE or LQ_PAULI_PHYSICAL. */
+int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
+                        int convention, double* out);
+
+/* ---- Gradients (src/optimization/gradient.rs) --------------------------- */
+/* ParameterShift::compute_gradient (gradient.rs:45-78) for an ansatz given as
+ * a gate template (param[] slots) of num_params parameters on n qubits,
+ * evaluated from |0...0> for the Pauli-sum observable.  energy_out (nullable)
+ * receives E(params).  mode: LQ_GRAD_SEQUENTIAL runs the 2P circuits one
+ * after another with per-gate kernels, as the reference does;
+ * LQ_GRAD_BATCHED runs all 2P+1 circuits together in tile passes. */
+#define LQ_GRAD_SEQUENTIAL 0
+#define LQ_GRAD_BATCHED 1
+int lq_param_shift_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, const double* params,
+                            int num_params, double shift, const uint8_t* paulis, const double* coeffs,
+                            int num_terms, int convention, int mode, double* energy_out, double* grad_out);
+
+/* ---- Optimizers and VQE (src/optimization/optimizer.rs, vqe.rs) -------- */
+typedef double (*lq_objective_fn)(const double* x, int dim, void* user);
+typedef void (*lq_gradient_fn)(const double* x, int dim, double* grad, void* user);
+/* OptimizerResult (src/optimization/optimizer.rs:8-15); the parameters are
+ * returned in place and the convergence history in a caller buffer. */
+typedef struct lq_opt_result {
+    double optimal_value;
+    int32_t num_iterations; /* length of the convergence history */
+    int32_t converged;
+} lq_opt_result;
+/* GradientDescent::minimize (optimizer.rs:56-107).  x: initial point in,
+ * optimal parameters out; history (nullable) holds max_iterations doubles. */
+int lq_gd_minimize(double learning_rate, int max_iterations, double tolerance, lq_objective_fn f, lq_gradient_fn g,
+                   void* user, double* x, int dim, double* history, lq_opt_result* result);
+/* Adam::minimize (optimizer.rs:139-223). */
+int lq_adam_minimize(double learning_rate, double beta1, double beta2, double epsilon, int max_iterations,
+                     double tolerance, lq_objective_fn f, lq_gradient_fn g, void* user, double* x, int dim,
+                     double* history, lq_opt_result* result);
+/* VQE::run (src/optimization/vqe.rs:55-99) with ParameterShift gradients
+ * (grad_mode LQ_GRAD_*) and GradientDescent (optimizer 0) or Adam
+ * (optimizer 1, beta1 0.9, beta2 0.999, eps 1e-8).  params: initial in,
+ * optimal out.  final_state (nullable) receives the state at the optimum. */
+int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+               const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+               int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
+               double* params, double* history, lq_state* final_state, lq_opt_result* result);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOGOSQ_HIP_H */

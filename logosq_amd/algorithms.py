@@ -1,0 +1,87 @@
+"""Algorithms on the GPU path: QFT circuits (src/algorithms/qft.rs:10-46),
+the seeded brickwork circuit modelled on rqc::generate_supremacy_circuit
+(src/algorithms/rqc.rs:236-268), and the documented input generators."""
+import math
+
+import numpy as np
+
+from .circuits import Circuit
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the seeded generator behind every
+    synthetic input; uniforms take the top 53 bits."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK64
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+
+    def normal(self):
+        # Box-Muller, one value per call
+        u1 = self.uniform()
+        u2 = self.uniform()
+        if u1 <= 0.0:
+            u1 = 1.0 / 9007199254740992.0
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def normal_state_vector(n, seed):
+    """psi_k = N(0,1) + i N(0,1) i.i.d. (unnormalised; State::new normalises)."""
+    r = SplitMix64(seed)
+    dim = 1 << n
+    v = np.empty(dim, dtype=np.complex128)
+    for k in range(dim):
+        re = r.normal()
+        im = r.normal()
+        v[k] = complex(re, im)
+    return v
+
+
+def qft_circuit(num_qubits):
+    """qft::create_circuit: dense-builder H, CPhaseGate(i, j, pi/2^(j-i)), swaps."""
+    c = Circuit(num_qubits).with_name("Quantum Fourier Transform")
+    for i in range(num_qubits):
+        c.h(i)
+        for j in range(i + 1, num_qubits):
+            c.controlled_phase(i, j, math.pi / float(1 << (j - i)))
+    for i in range(num_qubits // 2):
+        c.swap(i, num_qubits - i - 1)
+    return c
+
+
+def inverse_qft_circuit(num_qubits):
+    """qft::create_inverse_circuit (qft.rs:30-46)."""
+    c = Circuit(num_qubits).with_name("Inverse Quantum Fourier Transform")
+    for i in range(num_qubits // 2):
+        c.swap(i, num_qubits - i - 1)
+    for i in reversed(range(num_qubits)):
+        for j in range(i + 1, num_qubits):
+            c.controlled_phase(i, j, -math.pi / float(1 << (j - i)))
+        c.h(i)
+    return c
+
+
+def brickwork_circuit(num_qubits, depth, seed):
+    """Random brickwork: per layer one of RX/RY/RZ (uniform) with angle
+    U[0, 2pi) on every qubit, then CNOT(q, q+1) for q = layer%2, layer%2+2, ..."""
+    r = SplitMix64(seed)
+    c = Circuit(num_qubits).with_name("brickwork(d=%d)" % depth)
+    for layer in range(depth):
+        for q in range(num_qubits):
+            kind = min(2, int(r.uniform() * 3.0))
+            theta = 2.0 * math.pi * r.uniform()
+            (c.rx, c.ry, c.rz)[kind](q, theta)
+        for q in range(layer % 2, num_qubits - 1, 2):
+            c.cnot(q, q + 1)
+    return c

@@ -1,0 +1,229 @@
+// Classical optimisers and the VQE driver (host code): GradientDescent and
+// Adam (src/optimization/optimizer.rs:35-224) and VQE::run
+// (src/optimization/vqe.rs:55-99), with the objective and gradient evaluated
+// by the GPU path.
+#include <cmath>
+#include <vector>
+
+#include "lq_host.h"
+
+namespace lq {
+
+// f64::powi for a run-time exponent: binary exponentiation (LLVM powi).
+static double powi_f64(double a, int b) {
+    bool recip = b < 0;
+    unsigned e = recip ? (unsigned)(-b) : (unsigned)b;
+    double r = 1.0;
+    for (;;) {
+        if (e & 1u) r *= a;
+        e >>= 1;
+        if (e == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+static double l2(const std::vector<double>& g) {
+    double s = 0.0;
+    for (double v : g) s += v * v;
+    return std::sqrt(s);
+}
+
+struct Cb {
+    lq_objective_fn f;
+    lq_gradient_fn g;
+    void* user;
+};
+
+int gd_minimize(double lr, int max_it, double tol, const Cb& cb, double* x, int dim, double* history,
+                lq_opt_result* res) {
+    std::vector<double> grad(dim > 0 ? dim : 1);
+    int hist = 0;
+    bool converged = false;
+    for (int it = 0; it < max_it; ++it) {
+        double value = cb.f(x, dim, cb.user);
+        if (history) history[hist] = value;
+        ++hist;
+        cb.g(x, dim, grad.data(), cb.user);
+        if (l2(grad) < tol) {
+            converged = true;
+            break;
+        }
+        for (int i = 0; i < dim; ++i) x[i] -= lr * grad[i];
+    }
+    res->optimal_value = cb.f(x, dim, cb.user);
+    cb.g(x, dim, grad.data(), cb.user);
+    if (l2(grad) < tol) converged = true;
+    res->num_iterations = hist;
+    res->converged = converged ? 1 : 0;
+    return LQ_OK;
+}
+
+int adam_minimize(double lr, double b1, double b2, double eps, int max_it, double tol, const Cb& cb, double* x,
+                  int dim, double* history, lq_opt_result* res) {
+    std::vector<double> m(dim, 0.0), v(dim, 0.0), grad(dim > 0 ? dim : 1);
+    int hist = 0;
+    bool converged = false, have_prev = false;
+    double prev = 0.0;
+    for (int it = 1; it <= max_it; ++it) {
+        double value = cb.f(x, dim, cb.user);
+        if (history) history[hist] = value;
+        ++hist;
+        if (have_prev && std::fabs(prev - value) < tol) {
+            converged = true;
+            break;
+        }
+        prev = value;
+        have_prev = true;
+        cb.g(x, dim, grad.data(), cb.user);
+        if (l2(grad) < tol) {
+            converged = true;
+            break;
+        }
+        for (int i = 0; i < dim; ++i) {
+            m[i] = b1 * m[i] + (1.0 - b1) * grad[i];
+            v[i] = b2 * v[i] + (1.0 - b2) * grad[i] * grad[i];
+        }
+        double mc = 1.0 / (1.0 - powi_f64(b1, it));
+        double vc = 1.0 / (1.0 - powi_f64(b2, it));
+        double sn = 0.0;
+        for (int i = 0; i < dim; ++i) {
+            double mh = m[i] * mc, vh = v[i] * vc;
+            double step = lr * mh / (std::sqrt(vh) + eps);
+            x[i] -= step;
+            sn += step * step;
+        }
+        if (std::sqrt(sn) < tol) {
+            converged = true;
+            break;
+        }
+    }
+    res->optimal_value = cb.f(x, dim, cb.user);
+    cb.g(x, dim, grad.data(), cb.user);
+    if (l2(grad) < tol) converged = true;
+    res->num_iterations = hist;
+    res->converged = converged ? 1 : 0;
+    return LQ_OK;
+}
+
+// VQE objective / gradient on the GPU path.
+struct VqeCtx {
+    int n, dtype, num_gates, num_params, grad_mode;
+    const lq_gate* tmpl;
+    const PauliSum* ps;
+    double shift;
+    lq_state* st;
+    std::vector<lq_gate> sg;
+    const uint8_t* paulis;
+    const double* coeffs;
+    int num_terms, convention;
+    int rc;
+};
+
+int energy_sequential(lq_state* st, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
+                      const PauliSum& ps, std::vector<lq_gate>& scratch_gates, double* out);
+
+static double vqe_f(const double* x, int dim, void* user) {
+    VqeCtx* c = (VqeCtx*)user;
+    double e = 0.0;
+    if (c->rc) return NAN;
+    c->rc = energy_sequential(c->st, c->tmpl, c->num_gates, x, dim, *c->ps, c->sg, &e);
+    return c->rc ? NAN : e;
+}
+
+static void vqe_g(const double* x, int dim, double* grad, void* user) {
+    VqeCtx* c = (VqeCtx*)user;
+    if (c->rc) {
+        for (int i = 0; i < dim; ++i) grad[i] = NAN;
+        return;
+    }
+    c->rc = lq_param_shift_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->shift, c->paulis, c->coeffs,
+                                    c->num_terms, c->convention, c->grad_mode, nullptr, grad);
+    if (c->rc)
+        for (int i = 0; i < dim; ++i) grad[i] = NAN;
+}
+
+int check_template(int n, const lq_gate* tmpl, int num_gates, int num_params);
+
+}  // namespace lq
+
+using namespace lq;
+
+extern "C" {
+
+int lq_gd_minimize(double learning_rate, int max_iterations, double tolerance, lq_objective_fn f, lq_gradient_fn g,
+                   void* user, double* x, int dim, double* history, lq_opt_result* result) {
+    clear_error();
+    if (!f || !g || !result || dim < 0 || (dim > 0 && !x)) {
+        set_error("invalid optimiser arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    Cb cb{f, g, user};
+    return gd_minimize(learning_rate, max_iterations, tolerance, cb, x, dim, history, result);
+}
+
+int lq_adam_minimize(double learning_rate, double beta1, double beta2, double epsilon, int max_iterations,
+                     double tolerance, lq_objective_fn f, lq_gradient_fn g, void* user, double* x, int dim,
+                     double* history, lq_opt_result* result) {
+    clear_error();
+    if (!f || !g || !result || dim < 0 || (dim > 0 && !x)) {
+        set_error("invalid optimiser arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    Cb cb{f, g, user};
+    return adam_minimize(learning_rate, beta1, beta2, epsilon, max_iterations, tolerance, cb, x, dim, history,
+                         result);
+}
+
+int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+               const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+               int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
+               double* params, double* history, lq_state* final_state, lq_opt_result* result) {
+    clear_error();
+    if (!result || num_params < 0 || (num_params > 0 && !params)) {
+        set_error("invalid VQE arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    int rc = check_template(num_qubits, tmpl, num_gates, num_params);
+    if (rc) return rc;
+    PauliSum ps;
+    rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
+    if (rc) return rc;
+    VqeCtx c;
+    c.n = num_qubits;
+    c.dtype = dtype;
+    c.num_gates = num_gates;
+    c.num_params = num_params;
+    c.grad_mode = grad_mode;
+    c.tmpl = tmpl;
+    c.ps = &ps;
+    c.shift = shift;
+    c.paulis = paulis;
+    c.coeffs = coeffs;
+    c.num_terms = num_terms;
+    c.convention = convention;
+    c.rc = 0;
+    c.st = nullptr;
+    rc = lq_state_create(num_qubits, dtype, &c.st);
+    if (rc) return rc;
+    Cb cb{vqe_f, vqe_g, &c};
+    if (optimizer == 1)
+        rc = adam_minimize(learning_rate, 0.9, 0.999, 1e-8, max_iterations, tolerance, cb, params, num_params, history,
+                           result);
+    else
+        rc = gd_minimize(learning_rate, max_iterations, tolerance, cb, params, num_params, history, result);
+    if (!rc) rc = c.rc;
+    if (!rc && final_state) {
+        if (final_state->n != num_qubits || final_state->dtype != dtype) {
+            set_error("final_state has the wrong shape");
+            rc = LQ_ERR_DIMENSION_MISMATCH;
+        } else {
+            double e = 0.0;
+            rc = energy_sequential(final_state, tmpl, num_gates, params, num_params, ps, c.sg, &e);
+        }
+    }
+    lq_state_destroy(c.st);
+    return rc;
+}
+
+}  // extern "C"
