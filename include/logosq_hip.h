@@ -186,6 +186,17 @@ int lq_apply_dense(lq_state* st, const double* matrix, uint64_t dim);
  * Angles come from theta[] (param[] is ignored). */
 int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode);
 
+/* Tile geometry of the fused engine (no counterpart in the reference): tiles
+ * of 2^tile_bits amplitudes per dtype whose low_bits lowest bits are always
+ * contiguous; 0 keeps the default (12 / 13 / 5).  Used by the autotuner. */
+int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits);
+/* Planner introspection (no counterpart in the reference): the order in which
+ * the fused engine applies the gates (gate_order[k] = index of the k-th gate
+ * applied, count entries), the pass each lands in, and the number of passes.
+ * tile_bits / low_bits <= 0 select the defaults for dtype. */
+int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, int count, int tile_bits, int low_bits,
+                  int* gate_order, int* pass_index, int* num_passes);
+
 /* ---- Observables (src/optimization/observable.rs) ----------------------- */
 /* PauliObservable::expectation (observable.rs:228-231) =
  * sum_t coeff[t] * PauliTerm::expectation (observable.rs:95-147).

@@ -5,7 +5,7 @@
 #include <vector>
 
 #include "lq_common.h"
-#include "lq_host.h"
+#include "lq_fused.h"
 
 namespace lq {
 
@@ -58,156 +58,59 @@ int validate_gate(int n, const lq_gate* g) {
     return LQ_OK;
 }
 
-// 2x2 matrix of a single-qubit gate kind (host, f64, reference formulas).
-static void mat_rx(double th, double* m) {
-    double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-    double r[8] = {c, 0, 0, -s, 0, -s, c, 0};
-    std::memcpy(m, r, sizeof(r));
-}
-static void mat_ry(double th, double* m) {
-    double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-    double r[8] = {c, 0, -s, 0, s, 0, c, 0};
-    std::memcpy(m, r, sizeof(r));
-}
-static void mat_u3(double th, double ph, double la, double* m) {
-    // U3Gate (parameterized.rs:254-261)
-    double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-    double p[2];
-    m[0] = c;
-    m[1] = 0.0;
-    polar(s, la, p);
-    m[2] = -p[0];
-    m[3] = -p[1];
-    polar(s, ph, p);
-    m[4] = p[0];
-    m[5] = p[1];
-    polar(c, ph + la, p);
-    m[6] = p[0];
-    m[7] = p[1];
+// 2x2 matrices of the rotation gates (host, f64, the reference's formulas):
+// RX/CRX (parameterized.rs:70-74), RY/CRY (123-127), U3 (254-261).
+void gate_pair_matrix(int kind, const double* th, double* m) {
+    if (kind == LQ_GATE_RX || kind == LQ_GATE_CRX) {
+        double c = std::cos(th[0] / 2.0), s = std::sin(th[0] / 2.0);
+        double r[8] = {c, 0, 0, -s, 0, -s, c, 0};
+        std::memcpy(m, r, sizeof(r));
+    } else if (kind == LQ_GATE_RY || kind == LQ_GATE_CRY) {
+        double c = std::cos(th[0] / 2.0), s = std::sin(th[0] / 2.0);
+        double r[8] = {c, 0, -s, 0, s, 0, c, 0};
+        std::memcpy(m, r, sizeof(r));
+    } else {  // U3
+        double c = std::cos(th[0] / 2.0), s = std::sin(th[0] / 2.0);
+        double p[2];
+        m[0] = c;
+        m[1] = 0.0;
+        polar(s, th[2], p);
+        m[2] = -p[0];
+        m[3] = -p[1];
+        polar(s, th[1], p);
+        m[4] = p[0];
+        m[5] = p[1];
+        polar(c, th[1] + th[2], p);
+        m[6] = p[0];
+        m[7] = p[1];
+    }
 }
 
-// Per-gate dispatch.  Angles are taken from g->theta.
-int apply_gate_desc(lq_state* st, const lq_gate* g) {
-    const int n = st->n;
-    int rc = validate_gate(n, g);
-    if (rc) return rc;
-    const int* q = g->qubits;
-    const double* th = g->theta;
-    double m[8];
-    double d[8] = {1, 0, 1, 0, 1, 0, 1, 0};
-    switch (g->kind) {
-        case LQ_GATE_X:
-            rc = launch_pair(st, bitpos(n, q[0]), 0, m, kOpSwap);
-            break;
-        case LQ_GATE_Y:
-            rc = launch_pair(st, bitpos(n, q[0]), 0, m, kOpNegY);
-            break;
-        case LQ_GATE_Z:
-            d[0] = -1.0;
-            d[1] = 0.0;
-            rc = launch_diag(st, bitmask(n, q[0]), -1, -1, d);
-            break;
-        case LQ_GATE_H: {
-            double s = 1.0 / std::sqrt(2.0);  // 1.0 / SQRT_2 (single_qubit.rs:237)
-            double hm[8] = {s, 0, s, 0, s, 0, -s, 0};
-            rc = launch_pair(st, bitpos(n, q[0]), 0, hm, 3 /* Hadamard op */);
-            break;
+// Run primitives with the per-gate kernels, in order.
+int run_prims_per_gate(lq_state* st, const std::vector<Prim>& prims) {
+    for (const Prim& p : prims) {
+        int rc;
+        switch (p.kind) {
+            case kTPairMat: rc = launch_pair(st, p.b0, p.ctrl, p.m, kOpMatrix); break;
+            case kTPairSwap: rc = launch_pair(st, p.b0, p.ctrl, p.m, kOpSwap); break;
+            case kTPairNegY: rc = launch_pair(st, p.b0, p.ctrl, p.m, kOpNegY); break;
+            case kTPairH: rc = launch_pair(st, p.b0, p.ctrl, p.m, 3 /* Hadamard op */); break;
+            case kTDiag: rc = launch_diag(st, p.ctrl, p.sel0, p.sel1, p.m); break;
+            case kTSwap2: rc = launch_swap2(st, p.b0, p.b1); break;
+            case kTQuad: rc = launch_quad(st, p.b0, p.b1, p.m); break;
+            default: rc = normalize_device(st, nullptr); break;
         }
-        case LQ_GATE_S:
-            d[0] = 0.0;
-            d[1] = 1.0;
-            rc = launch_diag(st, bitmask(n, q[0]), -1, -1, d);
-            break;
-        case LQ_GATE_T:
-            polar(1.0, M_PI / 4.0, d);
-            rc = launch_diag(st, bitmask(n, q[0]), -1, -1, d);
-            break;
-        case LQ_GATE_RX:
-            mat_rx(th[0], m);
-            rc = launch_pair(st, bitpos(n, q[0]), 0, m, kOpMatrix);
-            break;
-        case LQ_GATE_RY:
-            mat_ry(th[0], m);
-            rc = launch_pair(st, bitpos(n, q[0]), 0, m, kOpMatrix);
-            break;
-        case LQ_GATE_RZ:
-            polar(1.0, -th[0] / 2.0, d);
-            polar(1.0, th[0] / 2.0, d + 2);
-            rc = launch_diag(st, 0, bitpos(n, q[0]), -1, d);
-            break;
-        case LQ_GATE_PHASE:
-            polar(1.0, th[0], d);
-            rc = launch_diag(st, bitmask(n, q[0]), -1, -1, d);
-            break;
-        case LQ_GATE_U3:
-            mat_u3(th[0], th[1], th[2], m);
-            rc = launch_pair(st, bitpos(n, q[0]), 0, m, kOpMatrix);
-            break;
-        case LQ_GATE_CNOT:
-            rc = launch_pair(st, bitpos(n, q[1]), bitmask(n, q[0]), m, kOpSwap);
-            break;
-        case LQ_GATE_SWAP:
-            rc = launch_swap2(st, bitpos(n, q[0]), bitpos(n, q[1]));
-            break;
-        case LQ_GATE_CZ:
-            d[0] = -1.0;
-            d[1] = 0.0;
-            rc = launch_diag(st, bitmask(n, q[0]) | bitmask(n, q[1]), -1, -1, d);
-            break;
-        case LQ_GATE_CPHASE:
-        case LQ_GATE_CP:
-            polar(1.0, th[0], d);
-            rc = launch_diag(st, bitmask(n, q[0]) | bitmask(n, q[1]), -1, -1, d);
-            break;
-        case LQ_GATE_CRX:
-            mat_rx(th[0], m);
-            rc = launch_pair(st, bitpos(n, q[1]), bitmask(n, q[0]), m, kOpMatrix);
-            break;
-        case LQ_GATE_CRY:
-            mat_ry(th[0], m);
-            rc = launch_pair(st, bitpos(n, q[1]), bitmask(n, q[0]), m, kOpMatrix);
-            break;
-        case LQ_GATE_CRZ:
-            polar(1.0, -th[0] / 2.0, d);
-            polar(1.0, th[0] / 2.0, d + 2);
-            rc = launch_diag(st, bitmask(n, q[0]), bitpos(n, q[1]), -1, d);
-            break;
-        case LQ_GATE_RXX:
-        case LQ_GATE_RYY: {
-            double M[32];
-            two_qubit_rot_matrix(g->kind, th[0], q[0], q[1], M);
-            rc = launch_quad(st, bitpos(n, q[0]), bitpos(n, q[1]), M);
-            break;
-        }
-        case LQ_GATE_RZZ:
-            // same parity (sub-index 0 or 3): e^{-i th/2}; different: e^{+i th/2}
-            polar(1.0, -th[0] / 2.0, d);
-            polar(1.0, th[0] / 2.0, d + 2);
-            polar(1.0, th[0] / 2.0, d + 4);
-            polar(1.0, -th[0] / 2.0, d + 6);
-            rc = launch_diag(st, 0, bitpos(n, q[0]), bitpos(n, q[1]), d);
-            break;
-        case LQ_GATE_TOFFOLI:
-            rc = launch_pair(st, bitpos(n, q[2]), bitmask(n, q[0]) | bitmask(n, q[1]), m, kOpSwap);
-            break;
-        case LQ_GATE_MATRIX1:
-            rc = launch_pair(st, bitpos(n, q[0]), 0, g->matrix, kOpMatrix);
-            if (!rc) rc = normalize_device(st, nullptr);
-            break;
-        case LQ_GATE_MATRIX2: {
-            int q1 = q[0] < q[1] ? q[0] : q[1], q2 = q[0] < q[1] ? q[1] : q[0];
-            rc = launch_quad(st, bitpos(n, q1), bitpos(n, q2), g->matrix);
-            if (!rc) rc = normalize_device(st, nullptr);
-            break;
-        }
-        default:
-            set_error("unknown gate kind %d", g->kind);
-            return LQ_ERR_INVALID_ARGUMENT;
+        if (rc) return rc;
     }
+    return LQ_OK;
+}
+
+// Per-gate dispatch (Gate::apply).  Angles are taken from g->theta.
+int apply_gate_desc(lq_state* st, const lq_gate* g) {
+    std::vector<Prim> prims;
+    int rc = gates_to_prims(st->n, g, 1, prims);
     if (rc) return rc;
-    if ((g->flags & LQ_GATE_FLAG_RENORMALIZE) && g->kind != LQ_GATE_MATRIX1 && g->kind != LQ_GATE_MATRIX2)
-        rc = normalize_device(st, nullptr);
-    return rc;
+    return run_prims_per_gate(st, prims);
 }
 
 // RXX / RYY as a 4x4 block with the reference's orientation rule: the
@@ -350,11 +253,10 @@ int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode) {
         if (rc) return rc;
     }
     if (mode == LQ_EXEC_PER_GATE) {
-        for (int i = 0; i < count; ++i) {
-            rc = apply_gate_desc(st, &gates[i]);
-            if (rc) return rc;
-        }
-        return LQ_OK;
+        std::vector<Prim> prims;
+        rc = gates_to_prims(st->n, gates, count, prims);
+        if (rc) return rc;
+        return run_prims_per_gate(st, prims);
     }
     if (mode == LQ_EXEC_FUSED) return execute_fused(st, gates, count);
     set_error("unknown execution mode %d", mode);

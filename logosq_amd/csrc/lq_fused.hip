@@ -1,15 +1,606 @@
-// Fused tile-pass execution (placeholder until the tile engine lands).
-#include "lq_host.h"
+// Fused tile-pass engine for Circuit::execute (src/circuits/mod.rs:352-375).
+//
+// A pass streams the state through LDS in tiles of 2^K amplitudes.  A tile
+// holds every combination of its K tile bits for one fixed value of the other
+// n-K bits: the low L bits (contiguous in memory, so loads and stores are
+// coalesced runs of 2^L amplitudes) plus K-L "high" bits chosen per pass.
+// Every gate whose non-diagonal target bits are tile bits is applied in LDS,
+// so one HBM read and one HBM write serve a whole run of gates.  Diagonal
+// gates (RZ, CZ, CP, phases; controls anywhere) need no tile bits: each
+// amplitude knows its global index.
+//
+// The planner walks the gate list in order and moves a gate into the current
+// pass when its target bits fit and it commutes with every gate deferred
+// before it (disjoint non-diagonal supports), so the result equals in-order
+// execution up to floating-point rounding.  MatrixGate's renormalisation
+// (src/gates/mod.rs:91) is a barrier: the pass ends, State::normalize runs,
+// planning continues.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "lq_fused.h"
 
 namespace lq {
 
-int execute_fused(lq_state* st, const lq_gate* gates, int count) {
-    (void)st;
-    (void)gates;
-    (void)count;
-    set_error("fused execution is not built yet");
-    return LQ_ERR_UNSUPPORTED;
+// ---------------------------------------------------------------------------------
+// gate -> primitive
+// ---------------------------------------------------------------------------------
+static inline uint64_t bmask(int n, int q) { return 1ull << (n - 1 - q); }
+static inline int bpos(int n, int q) { return n - 1 - q; }
+
+static void polar2(double r, double th, double* out) {
+    out[0] = r * std::cos(th);
+    out[1] = r * std::sin(th);
 }
+
+static Prim make_prim(int kind) {
+    Prim p;
+    std::memset(&p, 0, sizeof(p));
+    p.kind = kind;
+    p.b0 = p.b1 = -1;
+    p.sel0 = p.sel1 = -1;
+    return p;
+}
+
+static Prim pair_prim(int kind, int b, uint64_t ctrl, const double* m8) {
+    Prim p = make_prim(kind);
+    p.b0 = b;
+    p.ctrl = ctrl;
+    if (m8) std::memcpy(p.m, m8, 8 * sizeof(double));
+    p.nd = 1ull << b;
+    p.ds = ctrl;
+    return p;
+}
+
+static Prim diag_prim(uint64_t fixed, int sel0, int sel1, const double* d8) {
+    Prim p = make_prim(kTDiag);
+    p.ctrl = fixed;
+    p.sel0 = sel0;
+    p.sel1 = sel1;
+    std::memcpy(p.m, d8, 8 * sizeof(double));
+    p.nd = 0;
+    p.ds = fixed | (sel0 >= 0 ? 1ull << sel0 : 0) | (sel1 >= 0 ? 1ull << sel1 : 0);
+    return p;
+}
+
+int gates_to_prims(int n, const lq_gate* gates, int count, std::vector<Prim>& out) {
+    for (int gi = 0; gi < count; ++gi) {
+        const lq_gate* g = &gates[gi];
+        const size_t first = out.size();
+        int rc = validate_gate(n, g);
+        if (rc) return rc;
+        const int* q = g->qubits;
+        const double* th = g->theta;
+        double m[8];
+        double d[8] = {1, 0, 1, 0, 1, 0, 1, 0};
+        bool renorm = (g->flags & LQ_GATE_FLAG_RENORMALIZE) != 0;
+        switch (g->kind) {
+            case LQ_GATE_X: out.push_back(pair_prim(kTPairSwap, bpos(n, q[0]), 0, nullptr)); break;
+            case LQ_GATE_Y: out.push_back(pair_prim(kTPairNegY, bpos(n, q[0]), 0, nullptr)); break;
+            case LQ_GATE_Z:
+                d[0] = -1.0;
+                d[1] = 0.0;
+                out.push_back(diag_prim(bmask(n, q[0]), -1, -1, d));
+                break;
+            case LQ_GATE_H: {
+                double s = 1.0 / std::sqrt(2.0);  // single_qubit.rs:237
+                double hm[8] = {s, 0, s, 0, s, 0, -s, 0};
+                out.push_back(pair_prim(kTPairH, bpos(n, q[0]), 0, hm));
+                break;
+            }
+            case LQ_GATE_S:
+                d[0] = 0.0;
+                d[1] = 1.0;
+                out.push_back(diag_prim(bmask(n, q[0]), -1, -1, d));
+                break;
+            case LQ_GATE_T:
+                polar2(1.0, M_PI / 4.0, d);
+                out.push_back(diag_prim(bmask(n, q[0]), -1, -1, d));
+                break;
+            case LQ_GATE_RX:
+            case LQ_GATE_RY:
+            case LQ_GATE_U3:
+            case LQ_GATE_CRX:
+            case LQ_GATE_CRY: {
+                bool ctl = g->kind == LQ_GATE_CRX || g->kind == LQ_GATE_CRY;
+                gate_pair_matrix(g->kind, th, m);
+                out.push_back(pair_prim(kTPairMat, bpos(n, ctl ? q[1] : q[0]), ctl ? bmask(n, q[0]) : 0, m));
+                break;
+            }
+            case LQ_GATE_RZ:
+                polar2(1.0, -th[0] / 2.0, d);
+                polar2(1.0, th[0] / 2.0, d + 2);
+                out.push_back(diag_prim(0, bpos(n, q[0]), -1, d));
+                break;
+            case LQ_GATE_PHASE:
+                polar2(1.0, th[0], d);
+                out.push_back(diag_prim(bmask(n, q[0]), -1, -1, d));
+                break;
+            case LQ_GATE_CNOT: out.push_back(pair_prim(kTPairSwap, bpos(n, q[1]), bmask(n, q[0]), nullptr)); break;
+            case LQ_GATE_SWAP: {
+                Prim p = make_prim(kTSwap2);
+                p.b0 = bpos(n, q[0]);
+                p.b1 = bpos(n, q[1]);
+                p.nd = (1ull << p.b0) | (1ull << p.b1);
+                out.push_back(p);
+                break;
+            }
+            case LQ_GATE_CZ:
+                d[0] = -1.0;
+                d[1] = 0.0;
+                out.push_back(diag_prim(bmask(n, q[0]) | bmask(n, q[1]), -1, -1, d));
+                break;
+            case LQ_GATE_CPHASE:
+            case LQ_GATE_CP:
+                polar2(1.0, th[0], d);
+                out.push_back(diag_prim(bmask(n, q[0]) | bmask(n, q[1]), -1, -1, d));
+                break;
+            case LQ_GATE_CRZ:
+                polar2(1.0, -th[0] / 2.0, d);
+                polar2(1.0, th[0] / 2.0, d + 2);
+                out.push_back(diag_prim(bmask(n, q[0]), bpos(n, q[1]), -1, d));
+                break;
+            case LQ_GATE_RXX:
+            case LQ_GATE_RYY: {
+                Prim p = make_prim(kTQuad);
+                p.b0 = bpos(n, q[0]);
+                p.b1 = bpos(n, q[1]);
+                two_qubit_rot_matrix(g->kind, th[0], q[0], q[1], p.m);
+                p.nd = (1ull << p.b0) | (1ull << p.b1);
+                out.push_back(p);
+                break;
+            }
+            case LQ_GATE_RZZ:
+                polar2(1.0, -th[0] / 2.0, d);
+                polar2(1.0, th[0] / 2.0, d + 2);
+                polar2(1.0, th[0] / 2.0, d + 4);
+                polar2(1.0, -th[0] / 2.0, d + 6);
+                out.push_back(diag_prim(0, bpos(n, q[0]), bpos(n, q[1]), d));
+                break;
+            case LQ_GATE_TOFFOLI:
+                out.push_back(pair_prim(kTPairSwap, bpos(n, q[2]), bmask(n, q[0]) | bmask(n, q[1]), nullptr));
+                break;
+            case LQ_GATE_MATRIX1:
+                out.push_back(pair_prim(kTPairMat, bpos(n, q[0]), 0, g->matrix));
+                renorm = true;
+                break;
+            case LQ_GATE_MATRIX2: {
+                int q1 = std::min(q[0], q[1]), q2 = std::max(q[0], q[1]);
+                Prim p = make_prim(kTQuad);
+                p.b0 = bpos(n, q1);
+                p.b1 = bpos(n, q2);
+                std::memcpy(p.m, g->matrix, 32 * sizeof(double));
+                p.nd = (1ull << p.b0) | (1ull << p.b1);
+                out.push_back(p);
+                renorm = true;
+                break;
+            }
+            default:
+                set_error("unknown gate kind %d", g->kind);
+                return LQ_ERR_INVALID_ARGUMENT;
+        }
+        if (renorm) out.push_back(make_prim(-1));
+        for (size_t k = first; k < out.size(); ++k) out[k].src = gi;
+    }
+    return LQ_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// planner
+// ---------------------------------------------------------------------------------
+// Tile geometry: 2^12 complex128 / 2^13 complex64 amplitudes (64 KiB of LDS,
+// two tiles per CU) with 2^5-amplitude contiguous runs unless overridden.
+static int g_tile_c128 = 12, g_tile_c64 = 13, g_low = 5;
+
+int default_tile_bits(int n, int dtype) {
+    int k = dtype == LQ_C128 ? g_tile_c128 : g_tile_c64;
+    return std::min(k, n);
+}
+
+int default_low_bits(int n, int dtype) {
+    (void)dtype;
+    return std::min(g_low, n);
+}
+
+static TileOp to_tile_op(const Prim& p, int L, const int* hpos, int hb) {
+    auto local = [&](int b) -> int32_t {
+        if (b < 0) return -1;
+        if (b < L) return b;
+        for (int j = 0; j < hb; ++j)
+            if (hpos[j] == b) return L + j;
+        return -1;
+    };
+    TileOp t;
+    std::memset(&t, 0, sizeof(t));
+    t.kind = p.kind;
+    t.lb0 = local(p.b0);
+    t.lb1 = local(p.b1);
+    t.sel0 = p.sel0;
+    t.sel1 = p.sel1;
+    t.ctrl = p.ctrl;
+    std::memcpy(t.m, p.m, sizeof(t.m));
+    return t;
+}
+
+void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan) {
+    plan.n = n;
+    plan.steps.clear();
+    plan.ops.clear();
+    plan.op_src.clear();
+    plan.passes = 0;
+    K = std::min(K, n);
+    if (n <= K) {
+        L = n;  // one tile is the whole state: fully contiguous
+    }
+    L = std::min(L, K);
+    const int HB = K - L;
+    const uint64_t lowmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
+    size_t i = 0;
+    while (i < prims.size()) {
+        // segment up to the next renormalisation barrier
+        size_t j = i;
+        while (j < prims.size() && prims[j].kind >= 0) ++j;
+        std::vector<int> pending;
+        for (size_t k = i; k < j; ++k) pending.push_back((int)k);
+        while (!pending.empty()) {
+            uint64_t H = 0;
+            uint64_t dnd = 0, dds = 0;
+            std::vector<int> take, keep;
+            for (int idx : pending) {
+                const Prim& p = prims[idx];
+                uint64_t need = p.nd & ~lowmask;
+                bool blocked = (p.nd & (dnd | dds)) || (p.ds & dnd);
+                if (!blocked && __builtin_popcountll(H | need) <= HB) {
+                    H |= need;
+                    take.push_back(idx);
+                } else {
+                    dnd |= p.nd;
+                    dds |= p.ds;
+                    keep.push_back(idx);
+                }
+            }
+            // fill the high set: bits the deferred gates need first, then the
+            // lowest free bits (longer contiguous runs)
+            for (int idx : keep) {
+                uint64_t need = prims[idx].nd & ~lowmask & ~H;
+                if (need && __builtin_popcountll(H | need) <= HB) H |= need;
+            }
+            for (int b = L; b < n && __builtin_popcountll(H) < HB; ++b) H |= 1ull << b;
+            Step st;
+            st.normalize = false;
+            std::memset(&st.pass, 0, sizeof(st.pass));
+            st.pass.L = L;
+            st.pass.hb = 0;
+            for (int b = L; b < n; ++b)
+                if (H & (1ull << b)) st.pass.hpos[st.pass.hb++] = b;
+            st.pass.op_first = (int)plan.ops.size();
+            st.pass.op_count = (int)take.size();
+            st.pass.ntiles = 1ull << (n - L - st.pass.hb);
+            for (int idx : take) {
+                plan.ops.push_back(to_tile_op(prims[idx], L, st.pass.hpos, st.pass.hb));
+                plan.op_src.push_back(prims[idx].src);
+            }
+            plan.steps.push_back(st);
+            plan.passes++;
+            pending.swap(keep);
+        }
+        if (j < prims.size()) {
+            Step st;
+            std::memset(&st, 0, sizeof(st));
+            st.normalize = true;
+            plan.steps.push_back(st);
+        }
+        i = j + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// kernel
+// ---------------------------------------------------------------------------------
+constexpr int kTileThreads = 256;
+
+__device__ __forceinline__ uint32_t ins0(uint32_t k, int p) {
+    return ((k >> p) << (p + 1)) | (k & ((1u << p) - 1u));
+}
+
+template <typename V, typename T>
+__device__ __forceinline__ V cm(const double* m, int i, V a) {
+    return cmul((T)m[2 * i], (T)m[2 * i + 1], a);
+}
+
+template <int K, typename V, typename T>
+__global__ void __launch_bounds__(kTileThreads) k_tile(V* __restrict__ psi, TilePass p,
+                                                       const TileOp* __restrict__ ops, uint64_t ninst,
+                                                       uint64_t inst_stride, uint64_t ops_stride) {
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NT = kTileThreads;
+    constexpr int EPT = TS >= (uint32_t)NT ? (int)(TS / NT) : 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V* tile = reinterpret_cast<V*>(smem);
+    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem + TS * sizeof(V));
+    const int L = p.L;
+    const uint32_t lowmask = (1u << L) - 1u;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t h = tid; h < (1u << p.hb); h += NT) {
+        uint64_t o = 0;
+        for (int j = 0; j < p.hb; ++j)
+            if ((h >> j) & 1u) o |= 1ull << p.hpos[j];
+        hoff[h] = o;
+    }
+    __syncthreads();
+    const uint64_t total = ninst * p.ntiles;
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint64_t inst = w / p.ntiles;
+        const uint64_t t = w - inst * p.ntiles;
+        V* s = psi + inst * inst_stride;
+        const TileOp* op = ops + inst * ops_stride + p.op_first;
+        uint64_t base = t << L;
+        for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
+        // load the tile (all loads in flight before the LDS writes)
+        V r[EPT];
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            uint32_t e = tid + (uint32_t)j * NT;
+            if (e < TS) r[j] = s[base | (e & lowmask) | hoff[e >> L]];
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            uint32_t e = tid + (uint32_t)j * NT;
+            if (e < TS) tile[e] = r[j];
+        }
+        __syncthreads();
+        for (int o = 0; o < p.op_count;) {
+            const int kind = op[o].kind;
+            if (kind == kTDiag) {
+                int o2 = o + 1;
+                while (o2 < p.op_count && op[o2].kind == kTDiag) ++o2;
+                for (uint32_t e = tid; e < TS; e += NT) {
+                    const uint64_t g = base | (e & lowmask) | hoff[e >> L];
+                    V x = tile[e];
+                    for (int d = o; d < o2; ++d) {
+                        const TileOp& od = op[d];
+                        if ((g & od.ctrl) != od.ctrl) continue;
+                        int k = 0;
+                        if (od.sel0 >= 0) k |= (int)((g >> od.sel0) & 1ull);
+                        if (od.sel1 >= 0) k |= (int)((g >> od.sel1) & 1ull) << 1;
+                        x = cm<V, T>(od.m, k, x);
+                    }
+                    tile[e] = x;
+                }
+                o = o2;
+            } else if (kind <= kTPairH) {
+                const TileOp& po = op[o];
+                const int lb = po.lb0;
+                const uint64_t ctrl = po.ctrl;
+                for (uint32_t q = tid; q < TS / 2; q += NT) {
+                    const uint32_t e0 = ins0(q, lb);
+                    const uint32_t e1 = e0 | (1u << lb);
+                    if (ctrl) {
+                        const uint64_t g0 = base | (e0 & lowmask) | hoff[e0 >> L];
+                        if ((g0 & ctrl) != ctrl) continue;
+                    }
+                    V a0 = tile[e0], a1 = tile[e1];
+                    V o0, o1;
+                    if (kind == kTPairMat) {
+                        o0 = cadd(cm<V, T>(po.m, 0, a0), cm<V, T>(po.m, 1, a1));
+                        o1 = cadd(cm<V, T>(po.m, 2, a0), cm<V, T>(po.m, 3, a1));
+                    } else if (kind == kTPairSwap) {
+                        o0 = a1;
+                        o1 = a0;
+                    } else if (kind == kTPairNegY) {
+                        o0 = cmul((T)0, (T)1, a1);
+                        o1 = cmul((T)0, (T)-1, a0);
+                    } else {
+                        V sm, df;
+                        sm.x = a0.x + a1.x;
+                        sm.y = a0.y + a1.y;
+                        df.x = a0.x - a1.x;
+                        df.y = a0.y - a1.y;
+                        o0 = cmul((T)po.m[0], (T)0, sm);
+                        o1 = cmul((T)po.m[0], (T)0, df);
+                    }
+                    tile[e0] = o0;
+                    tile[e1] = o1;
+                }
+                ++o;
+            } else if (kind == kTSwap2) {
+                const TileOp& po = op[o];
+                const int lo = min(po.lb0, po.lb1), hi = max(po.lb0, po.lb1);
+                for (uint32_t q = tid; q < TS / 4; q += NT) {
+                    const uint32_t e = ins0(ins0(q, lo), hi) | (1u << po.lb0);
+                    const uint32_t f = e ^ (1u << po.lb0) ^ (1u << po.lb1);
+                    V x = tile[e];
+                    tile[e] = tile[f];
+                    tile[f] = x;
+                }
+                ++o;
+            } else {  // kTQuad
+                const TileOp& po = op[o];
+                const int lo = min(po.lb0, po.lb1), hi = max(po.lb0, po.lb1);
+                const uint32_t ma = 1u << po.lb0, mb = 1u << po.lb1;
+                for (uint32_t q = tid; q < TS / 4; q += NT) {
+                    const uint32_t e = ins0(ins0(q, lo), hi);
+                    const uint32_t ix[4] = {e, e | mb, e | ma, e | ma | mb};
+                    V a[4];
+#pragma unroll
+                    for (int s2 = 0; s2 < 4; ++s2) a[s2] = tile[ix[s2]];
+#pragma unroll
+                    for (int r2 = 0; r2 < 4; ++r2) {
+                        V acc = cm<V, T>(po.m, 4 * r2, a[0]);
+#pragma unroll
+                        for (int c = 1; c < 4; ++c) acc = cadd(acc, cm<V, T>(po.m, 4 * r2 + c, a[c]));
+                        tile[ix[r2]] = acc;
+                    }
+                }
+                ++o;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            uint32_t e = tid + (uint32_t)j * NT;
+            if (e < TS) s[base | (e & lowmask) | hoff[e >> L]] = tile[e];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// launcher
+// ---------------------------------------------------------------------------------
+typedef void (*tile_kernel_t)(void*, TilePass, const TileOp*, uint64_t, uint64_t, uint64_t);
+
+template <int K, typename V, typename T>
+static void launch_k(void* psi, const TilePass& p, const TileOp* ops, uint64_t ninst, uint64_t stride,
+                     uint64_t ostride, unsigned grid, size_t lds) {
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)k_tile<K, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds + 4096);
+    });
+    hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(kTileThreads), lds, stream(), (V*)psi, p, ops, ninst,
+                       stride, ostride);
+}
+
+template <typename V, typename T>
+static int dispatch_k(int K, void* psi, const TilePass& p, const TileOp* ops, uint64_t ninst, uint64_t stride,
+                      uint64_t ostride, unsigned grid, size_t lds) {
+    switch (K) {
+#define LQ_K(k) \
+    case k: launch_k<k, V, T>(psi, p, ops, ninst, stride, ostride, grid, lds); break;
+        LQ_K(1) LQ_K(2) LQ_K(3) LQ_K(4) LQ_K(5) LQ_K(6) LQ_K(7) LQ_K(8) LQ_K(9) LQ_K(10) LQ_K(11) LQ_K(12)
+        LQ_K(13) LQ_K(14)
+#undef LQ_K
+        default:
+            set_error("tile of 2^%d amplitudes is not supported", K);
+            return LQ_ERR_UNSUPPORTED;
+    }
+    return LQ_OK;
+}
+
+int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
+                     uint64_t inst_stride, uint64_t ops_stride) {
+    const int K = p.L + p.hb;
+    const size_t amp = dtype == LQ_C128 ? 16 : 8;
+    const size_t lds = (amp << K) + (sizeof(uint64_t) << p.hb);
+    uint64_t total = ninst * p.ntiles;
+    unsigned grid = (unsigned)std::min<uint64_t>(total, 4096);
+    int rc;
+    if (dtype == LQ_C128)
+        rc = dispatch_k<double2, double>(K, psi, p, d_ops, ninst, inst_stride, ops_stride, grid, lds);
+    else
+        rc = dispatch_k<float2, float>(K, psi, p, d_ops, ninst, inst_stride, ops_stride, grid, lds);
+    if (rc) return rc;
+    LQ_LAUNCH_CHECK();
+    return LQ_OK;
+}
+
+// Device copy of a plan's ops (grown on demand; reused across calls).
+static TileOp* g_ops_dev = nullptr;
+static size_t g_ops_cap = 0;
+static TileOp* g_ops_host = nullptr;
+
+static int upload_ops(const std::vector<TileOp>& ops, TileOp** out) {
+    size_t need = std::max<size_t>(ops.size(), 1);
+    if (need > g_ops_cap) {
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        if (g_ops_dev) (void)hipFree(g_ops_dev);
+        if (g_ops_host) (void)hipHostFree(g_ops_host);
+        g_ops_dev = nullptr;
+        g_ops_host = nullptr;
+        size_t cap = std::max<size_t>(need, 1024);
+        LQ_HIP_CHECK(hipMalloc((void**)&g_ops_dev, cap * sizeof(TileOp)));
+        LQ_HIP_CHECK(hipHostMalloc((void**)&g_ops_host, cap * sizeof(TileOp), hipHostMallocDefault));
+        g_ops_cap = cap;
+    } else {
+        // the pinned staging buffer may still feed an earlier copy
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+    }
+    std::memcpy(g_ops_host, ops.data(), ops.size() * sizeof(TileOp));
+    LQ_HIP_CHECK(hipMemcpyAsync(g_ops_dev, g_ops_host, ops.size() * sizeof(TileOp), hipMemcpyHostToDevice, stream()));
+    *out = g_ops_dev;
+    return LQ_OK;
+}
+
+int run_plan(lq_state* st, const Plan& plan, const TileOp* d_ops) {
+    for (const Step& s : plan.steps) {
+        int rc = s.normalize ? normalize_device(st, nullptr)
+                             : launch_tile_pass(st->d_amps, st->dtype, s.pass, d_ops, 1, 0, 0);
+        if (rc) return rc;
+    }
+    return LQ_OK;
+}
+
+int execute_fused(lq_state* st, const lq_gate* gates, int count) {
+    std::vector<Prim> prims;
+    int rc = gates_to_prims(st->n, gates, count, prims);
+    if (rc) return rc;
+    Plan plan;
+    plan_passes(st->n, prims, default_tile_bits(st->n, st->dtype), default_low_bits(st->n, st->dtype), plan);
+    if (plan.ops.empty() && plan.steps.empty()) return LQ_OK;
+    TileOp* d_ops = nullptr;
+    rc = upload_ops(plan.ops, &d_ops);
+    if (rc) return rc;
+    return run_plan(st, plan, d_ops);
+}
+
+}  // namespace lq
+
+using namespace lq;
+
+extern "C" int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits) {
+    clear_error();
+    int t1 = tile_bits_c128 > 0 ? tile_bits_c128 : 12;
+    int t2 = tile_bits_c64 > 0 ? tile_bits_c64 : 13;
+    int lo = low_bits > 0 ? low_bits : 5;
+    // LDS: 16 B * 2^t1 and 8 B * 2^t2 must fit the 160 KiB per CU
+    if (t1 > 13 || t2 > 14 || lo > t1 || lo > t2 || t1 < 1 || t2 < 1) {
+        set_error("tile config out of range (c128 <= 13, c64 <= 14, low <= tile)");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    g_tile_c128 = t1;
+    g_tile_c64 = t2;
+    g_low = lo;
+    return LQ_OK;
+}
+
+extern "C" int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, int count, int tile_bits,
+                             int low_bits, int* gate_order, int* pass_index, int* num_passes) {
+    clear_error();
+    if (num_qubits < 1 || num_qubits > kMaxQubits || count < 0 || (count > 0 && (!gates || !gate_order)) ||
+        !num_passes) {
+        set_error("invalid arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<Prim> prims;
+    int rc = gates_to_prims(num_qubits, gates, count, prims);
+    if (rc) return rc;
+    Plan plan;
+    int K = tile_bits > 0 ? tile_bits : default_tile_bits(num_qubits, dtype);
+    int L = low_bits > 0 ? low_bits : default_low_bits(num_qubits, dtype);
+    if (K > 14 || L > K) {
+        set_error("tile_bits must be <= 14 and low_bits <= tile_bits");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    plan_passes(num_qubits, prims, K, L, plan);
+    int k = 0, pass = 0;
+    for (const Step& s : plan.steps) {
+        if (s.normalize) continue;
+        for (int o = 0; o < s.pass.op_count; ++o) {
+            if (k >= count) break;
+            gate_order[k] = plan.op_src[s.pass.op_first + o];
+            if (pass_index) pass_index[k] = pass;
+            ++k;
+        }
+        ++pass;
+    }
+    *num_passes = plan.passes;
+    return LQ_OK;
+}
+
+namespace lq {
 
 int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
                         double shift, const PauliSum& ps, double* energy_out, double* grad_out) {

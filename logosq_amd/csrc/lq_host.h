@@ -12,6 +12,7 @@ int gate_arity(int kind);
 int validate_gate(int n, const lq_gate* g);
 int apply_gate_desc(lq_state* st, const lq_gate* g);
 void two_qubit_rot_matrix(int kind, double theta, int q1, int q2, double* M);
+void gate_pair_matrix(int kind, const double* th, double* m);
 
 // Fused (LDS tile pass) execution of a gate list (lq_fused.hip).
 int execute_fused(lq_state* st, const lq_gate* gates, int count);

@@ -1,0 +1,82 @@
+"""GPU parity of the fused tile engine (Circuit::execute in LDS tile passes)
+against the oracle's in-order execution: complex128 <= 1e-12, complex64
+<= 2e-6 (f32 rounding), on random circuits of every gate kind, several tile
+geometries, and states from one tile to many tiles."""
+import numpy as np
+import pytest
+
+import oracle
+import logosq_amd as lq
+from logosq_amd import abi
+from tests import bruteforce as bf
+from tests.test_planner import random_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def run(psi, n, gates, dtype, mode):
+    st = lq.State(n, dtype)
+    st.set_vector(psi)
+    lq._lib.call("lq_execute", st.handle, abi.gate_array(gates), len(gates), mode)
+    return st.vector()
+
+
+@pytest.fixture
+def tile_config():
+    yield lambda a, b, c: lq._lib.call("lq_set_fused_config", a, b, c)
+    lq._lib.call("lq_set_fused_config", 0, 0, 0)
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 12, 13, 15, 18])
+def test_fused_random_circuits(hip, n):
+    say("fused n=%d" % n)
+    rng = np.random.default_rng(500 + n)
+    gates = random_circuit(n, 150, rng)
+    psi = bf.random_state(n, rng)
+    want = oracle.execute(psi, n, gates)
+    got = run(psi, n, gates, lq.C128, abi.EXEC_FUSED)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    got64 = run(psi, n, gates, lq.C64, abi.EXEC_FUSED)
+    assert np.max(np.abs(got64 - want)) <= 2e-6
+
+
+@pytest.mark.parametrize("tile,low", [(7, 2), (9, 4), (13, 6)])
+def test_fused_tile_geometries(hip, tile_config, tile, low):
+    say("fused geometry tile=%d low=%d" % (tile, low))
+    tile_config(tile, tile, low)
+    n = 16
+    rng = np.random.default_rng(tile * 31 + low)
+    gates = random_circuit(n, 120, rng, with_dense=False)
+    psi = bf.random_state(n, rng)
+    want = oracle.execute(psi, n, gates)
+    assert np.max(np.abs(run(psi, n, gates, lq.C128, abi.EXEC_FUSED) - want)) <= 1e-12
+    assert np.max(np.abs(run(psi, n, gates, lq.C64, abi.EXEC_FUSED) - want)) <= 2e-6
+
+
+def test_fused_brickwork_22(hip):
+    """The brickwork generator at 22 qubits, depth 20: fused vs per-gate on the
+    GPU and vs the oracle."""
+    say("fused brickwork 22")
+    n = 22
+    circ = lq.algorithms.brickwork_circuit(n, 20, 2)
+    d = circ.descriptors()
+    psi = oracle.zero_state(n)
+    want = oracle.execute(psi, n, d)
+    got = run(psi, n, d, lq.C128, abi.EXEC_FUSED)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    per = run(psi, n, d, lq.C128, abi.EXEC_PER_GATE)
+    assert np.max(np.abs(per - want)) <= 1e-12
+
+
+def test_fused_qft_12(hip):
+    say("fused qft12")
+    n = 12
+    psi0 = lq.algorithms.normal_state_vector(n, 1)
+    st = lq.State.new(psi0)
+    lq.algorithms.qft_circuit(n).execute(st, abi.EXEC_FUSED)
+    ref = oracle.execute(oracle.normalize(psi0)[0], n, oracle.qft_gates(n))
+    assert np.max(np.abs(st.vector() - ref)) <= 1e-12
