@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--cpu-qubits", type=int, default=30)
     ap.add_argument("--cpu-gates", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
 
 
@@ -121,6 +122,12 @@ def main():
     dtype = lq.C128 if a.dtype == "c128" else lq.C64
     c = 16 if dtype == lq.C128 else 8
     mode = abi.EXEC_PER_GATE if a.mode == "per_gate" else abi.EXEC_FUSED
+    tune = None
+    if not a.no_tune:
+        buf = ctypes.create_string_buffer(8192)
+        lq._lib.call("lq_autotune", a.qubits, dtype, buf, 8192)
+        tune = json.loads(buf.value.decode())
+        print("bench: autotune %s %s" % (tune["gate_choice"], tune["tile_choice"]), file=sys.stderr, flush=True)
     circ = lq.algorithms.brickwork_circuit(a.qubits, a.depth, a.seed)
     descs = circ.descriptors()
     arr = abi.gate_array(descs)
@@ -196,7 +203,7 @@ def main():
         "config": {"workload": "random brickwork circuit, %d qubits, depth %d, %s, seed %d, %s execution"
                                % (a.qubits, a.depth, a.dtype, a.seed, a.mode.replace("_", "-")),
                    "qubits": a.qubits, "depth": a.depth, "gates": len(descs),
-                   "passes": passes, "hbm_bytes_per_step": bytes_moved,
+                   "passes": passes, "autotune": tune, "hbm_bytes_per_step": bytes_moved,
                    "gate_algorithmic_bytes_per_step": bytes_gates, "final_norm_sqr": norm2,
                    "per_gate_kernels": {"rx_pair_GBps": round(rx_gbs, 1), "rx_frac": round(rx_gbs / HBM_PEAK_GBS, 4),
                                         "cnot_GBps": round(cn_gbs, 1), "cnot_frac": round(cn_gbs / HBM_PEAK_GBS, 4)}},

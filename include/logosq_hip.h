@@ -190,6 +190,14 @@ int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode);
  * of 2^tile_bits amplitudes per dtype whose low_bits lowest bits are always
  * contiguous; 0 keeps the default (12 / 13 / 5).  Used by the autotuner. */
 int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits);
+/* Per-gate kernel variant (no counterpart in the reference): work units per
+ * wavefront (2, 4 or 8) and nontemporal global access on/off. */
+int lq_set_gate_kernel_config(int unroll, int nontemporal);
+/* Autotuner (no counterpart in the reference): times the per-gate kernel
+ * variants and the fused tile geometries on a num_qubits state of dtype
+ * (allocated for the call), keeps the fastest of each, and writes a JSON
+ * report of every candidate into report (nullable, report_len bytes). */
+int lq_autotune(int num_qubits, int dtype, char* report, size_t report_len);
 /* Planner introspection (no counterpart in the reference): the order in which
  * the fused engine applies the gates (gate_order[k] = index of the k-th gate
  * applied, count entries), the pass each lands in, and the number of passes.

@@ -54,6 +54,8 @@ SIGNATURES = {
     "lq_apply_dense": [c_void_p, P_double, c_u64],
     "lq_execute": [c_void_p, P_gate, c_int, c_int],
     "lq_set_fused_config": [c_int, c_int, c_int],
+    "lq_set_gate_kernel_config": [c_int, c_int],
+    "lq_autotune": [c_int, c_int, ctypes.c_char_p, ctypes.c_size_t],
     "lq_plan_fused": [c_int, c_int, P_gate, c_int, c_int, c_int, P_int, P_int, P_int],
     "lq_expect_pauli_sum": [c_void_p, P_u8, P_double, c_int, c_int, P_double],
     "lq_param_shift_gradient": [c_int, c_int, P_gate, c_int, P_double, c_int, c_double, P_u8, P_double, c_int,

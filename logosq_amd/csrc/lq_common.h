@@ -134,6 +134,9 @@ int launch_swap2(lq_state* st, int bit1, int bit2);
 int launch_quad(lq_state* st, int bit_a, int bit_b, const double m[32]);
 int launch_dense(lq_state* st, const double* d_matrix, uint64_t dim);
 
+// Per-gate kernel variant (work units per wave, nontemporal access).
+int set_gate_kernel_config(int unroll, int nontemporal);
+
 // Reductions (lq_state.hip).
 int reduce_norm_sqr(const lq_state* st, double* d_out);    // device scalar
 int normalize_device(lq_state* st, double* norm_out);      // norm_out host, nullable

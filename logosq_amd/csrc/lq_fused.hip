@@ -310,19 +310,35 @@ __device__ __forceinline__ V cm(const double* m, int i, V a) {
     return cmul((T)m[2 * i], (T)m[2 * i + 1], a);
 }
 
+// Select one of four uniform phases by two per-lane bits without indexing
+// memory with a lane-dependent value.
+template <typename V, typename T>
+__device__ __forceinline__ V apply_phase(const double* m, int b0, int b1, V x) {
+    const T r0 = (T)m[0], i0 = (T)m[1], r1 = (T)m[2], i1 = (T)m[3];
+    const T r2 = (T)m[4], i2 = (T)m[5], r3 = (T)m[6], i3 = (T)m[7];
+    T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
+    T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
+    return cmul(pr, pi, x);
+}
+
+// Threads per tile block: at most 16 amplitudes per thread.
+__host__ __device__ constexpr int tile_threads(int K) { return K <= 12 ? 256 : (1 << (K - 4)); }
+
 template <int K, typename V, typename T>
-__global__ void __launch_bounds__(kTileThreads) k_tile(V* __restrict__ psi, TilePass p,
-                                                       const TileOp* __restrict__ ops, uint64_t ninst,
-                                                       uint64_t inst_stride, uint64_t ops_stride) {
+__global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, TilePass p,
+                                                         const TileOp* __restrict__ ops, uint64_t ninst,
+                                                         uint64_t inst_stride, uint64_t ops_stride) {
     constexpr uint32_t TS = 1u << K;
-    constexpr int NT = kTileThreads;
-    constexpr int EPT = TS >= (uint32_t)NT ? (int)(TS / NT) : 1;
+    constexpr int NT = tile_threads(K);
+    constexpr bool FULL = TS >= (uint32_t)NT;  // every thread owns TS/NT elements
+    constexpr int EPT = FULL ? (int)(TS / NT) : 1;
     extern __shared__ __align__(16) unsigned char smem[];
     V* tile = reinterpret_cast<V*>(smem);
     uint64_t* hoff = reinterpret_cast<uint64_t*>(smem + TS * sizeof(V));
     const int L = p.L;
     const uint32_t lowmask = (1u << L) - 1u;
     const uint32_t tid = threadIdx.x;
+    const bool own = FULL || tid < TS;
     for (uint32_t h = tid; h < (1u << p.hb); h += NT) {
         uint64_t o = 0;
         for (int j = 0; j < p.hb; ++j)
@@ -338,54 +354,78 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(V* __restrict__ psi, Tile
         const TileOp* op = ops + inst * ops_stride + p.op_first;
         uint64_t base = t << L;
         for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
-        // load the tile (all loads in flight before the LDS writes)
-        V r[EPT];
+        // global index of element e of this tile
+        auto gidx = [&](uint32_t e) -> uint64_t { return base | (e & lowmask) | hoff[e >> L]; };
+        // load: every load is in flight before the first LDS write
+        if (own) {
+            V r[EPT];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            uint32_t e = tid + (uint32_t)j * NT;
-            if (e < TS) r[j] = s[base | (e & lowmask) | hoff[e >> L]];
-        }
+            for (int j = 0; j < EPT; ++j) r[j] = s[gidx(tid + (uint32_t)j * NT)];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            uint32_t e = tid + (uint32_t)j * NT;
-            if (e < TS) tile[e] = r[j];
+            for (int j = 0; j < EPT; ++j) tile[tid + (uint32_t)j * NT] = r[j];
         }
         __syncthreads();
         for (int o = 0; o < p.op_count;) {
             const int kind = op[o].kind;
             if (kind == kTDiag) {
+                // a run of diagonal ops: elements stay in registers across the run
                 int o2 = o + 1;
                 while (o2 < p.op_count && op[o2].kind == kTDiag) ++o2;
-                for (uint32_t e = tid; e < TS; e += NT) {
-                    const uint64_t g = base | (e & lowmask) | hoff[e >> L];
-                    V x = tile[e];
-                    for (int d = o; d < o2; ++d) {
-                        const TileOp& od = op[d];
-                        if ((g & od.ctrl) != od.ctrl) continue;
-                        int k = 0;
-                        if (od.sel0 >= 0) k |= (int)((g >> od.sel0) & 1ull);
-                        if (od.sel1 >= 0) k |= (int)((g >> od.sel1) & 1ull) << 1;
-                        x = cm<V, T>(od.m, k, x);
+                if (own) {
+                    // real and imaginary parts in separate scalar arrays so the
+                    // run stays in registers
+                    T xr[EPT], xi[EPT];
+#pragma unroll
+                    for (int j = 0; j < EPT; ++j) {
+                        V v = tile[tid + (uint32_t)j * NT];
+                        xr[j] = v.x;
+                        xi[j] = v.y;
                     }
-                    tile[e] = x;
+                    for (int d = o; d < o2; ++d) {
+                        const uint64_t ctrl = op[d].ctrl;
+                        const int s0 = op[d].sel0, s1 = op[d].sel1;
+                        const T r0 = (T)op[d].m[0], i0 = (T)op[d].m[1], r1 = (T)op[d].m[2], i1 = (T)op[d].m[3];
+                        const T r2 = (T)op[d].m[4], i2 = (T)op[d].m[5], r3 = (T)op[d].m[6], i3 = (T)op[d].m[7];
+#pragma unroll
+                        for (int j = 0; j < EPT; ++j) {
+                            const uint64_t g = gidx(tid + (uint32_t)j * NT);
+                            const bool on = (g & ctrl) == ctrl;
+                            const bool b0 = s0 >= 0 && ((g >> s0) & 1ull);
+                            const bool b1 = s1 >= 0 && ((g >> s1) & 1ull);
+                            const T pr = on ? (b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0)) : (T)1;
+                            const T pi = on ? (b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0)) : (T)0;
+                            const T yr = pr * xr[j] - pi * xi[j];
+                            const T yi = pr * xi[j] + pi * xr[j];
+                            xr[j] = on ? yr : xr[j];
+                            xi[j] = on ? yi : xi[j];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < EPT; ++j) {
+                        V v;
+                        v.x = xr[j];
+                        v.y = xi[j];
+                        tile[tid + (uint32_t)j * NT] = v;
+                    }
                 }
                 o = o2;
             } else if (kind <= kTPairH) {
-                const TileOp& po = op[o];
-                const int lb = po.lb0;
-                const uint64_t ctrl = po.ctrl;
-                for (uint32_t q = tid; q < TS / 2; q += NT) {
+                const int lb = op[o].lb0;
+                const uint64_t ctrl = op[o].ctrl;
+                const T m0 = (T)op[o].m[0], m1 = (T)op[o].m[1], m2 = (T)op[o].m[2], m3 = (T)op[o].m[3];
+                const T m4 = (T)op[o].m[4], m5 = (T)op[o].m[5], m6 = (T)op[o].m[6], m7 = (T)op[o].m[7];
+#pragma unroll
+                for (int j = 0; j < (FULL ? EPT / 2 : 1); ++j) {
+                    const uint32_t q = tid + (uint32_t)j * NT;
+                    if (!FULL && q >= TS / 2) continue;
                     const uint32_t e0 = ins0(q, lb);
                     const uint32_t e1 = e0 | (1u << lb);
-                    if (ctrl) {
-                        const uint64_t g0 = base | (e0 & lowmask) | hoff[e0 >> L];
-                        if ((g0 & ctrl) != ctrl) continue;
-                    }
+                    if (ctrl && (gidx(e0) & ctrl) != ctrl) continue;
                     V a0 = tile[e0], a1 = tile[e1];
                     V o0, o1;
                     if (kind == kTPairMat) {
-                        o0 = cadd(cm<V, T>(po.m, 0, a0), cm<V, T>(po.m, 1, a1));
-                        o1 = cadd(cm<V, T>(po.m, 2, a0), cm<V, T>(po.m, 3, a1));
+                        o0 = cadd(cmul(m0, m1, a0), cmul(m2, m3, a1));
+                        o1 = cadd(cmul(m4, m5, a0), cmul(m6, m7, a1));
                     } else if (kind == kTPairSwap) {
                         o0 = a1;
                         o1 = a0;
@@ -398,19 +438,19 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(V* __restrict__ psi, Tile
                         sm.y = a0.y + a1.y;
                         df.x = a0.x - a1.x;
                         df.y = a0.y - a1.y;
-                        o0 = cmul((T)po.m[0], (T)0, sm);
-                        o1 = cmul((T)po.m[0], (T)0, df);
+                        o0 = cmul(m0, (T)0, sm);
+                        o1 = cmul(m0, (T)0, df);
                     }
                     tile[e0] = o0;
                     tile[e1] = o1;
                 }
                 ++o;
             } else if (kind == kTSwap2) {
-                const TileOp& po = op[o];
-                const int lo = min(po.lb0, po.lb1), hi = max(po.lb0, po.lb1);
+                const int lb0 = op[o].lb0, lb1 = op[o].lb1;
+                const int lo = min(lb0, lb1), hi = max(lb0, lb1);
                 for (uint32_t q = tid; q < TS / 4; q += NT) {
-                    const uint32_t e = ins0(ins0(q, lo), hi) | (1u << po.lb0);
-                    const uint32_t f = e ^ (1u << po.lb0) ^ (1u << po.lb1);
+                    const uint32_t e = ins0(ins0(q, lo), hi) | (1u << lb0);
+                    const uint32_t f = e ^ (1u << lb0) ^ (1u << lb1);
                     V x = tile[e];
                     tile[e] = tile[f];
                     tile[f] = x;
@@ -422,26 +462,30 @@ __global__ void __launch_bounds__(kTileThreads) k_tile(V* __restrict__ psi, Tile
                 const uint32_t ma = 1u << po.lb0, mb = 1u << po.lb1;
                 for (uint32_t q = tid; q < TS / 4; q += NT) {
                     const uint32_t e = ins0(ins0(q, lo), hi);
-                    const uint32_t ix[4] = {e, e | mb, e | ma, e | ma | mb};
-                    V a[4];
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) a[s2] = tile[ix[s2]];
-#pragma unroll
-                    for (int r2 = 0; r2 < 4; ++r2) {
-                        V acc = cm<V, T>(po.m, 4 * r2, a[0]);
-#pragma unroll
-                        for (int c = 1; c < 4; ++c) acc = cadd(acc, cm<V, T>(po.m, 4 * r2 + c, a[c]));
-                        tile[ix[r2]] = acc;
-                    }
+                    V a0 = tile[e], a1 = tile[e | mb], a2 = tile[e | ma], a3 = tile[e | ma | mb];
+                    V y0 = cadd(cadd(cm<V, T>(po.m, 0, a0), cm<V, T>(po.m, 1, a1)),
+                                cadd(cm<V, T>(po.m, 2, a2), cm<V, T>(po.m, 3, a3)));
+                    V y1 = cadd(cadd(cm<V, T>(po.m, 4, a0), cm<V, T>(po.m, 5, a1)),
+                                cadd(cm<V, T>(po.m, 6, a2), cm<V, T>(po.m, 7, a3)));
+                    V y2 = cadd(cadd(cm<V, T>(po.m, 8, a0), cm<V, T>(po.m, 9, a1)),
+                                cadd(cm<V, T>(po.m, 10, a2), cm<V, T>(po.m, 11, a3)));
+                    V y3 = cadd(cadd(cm<V, T>(po.m, 12, a0), cm<V, T>(po.m, 13, a1)),
+                                cadd(cm<V, T>(po.m, 14, a2), cm<V, T>(po.m, 15, a3)));
+                    tile[e] = y0;
+                    tile[e | mb] = y1;
+                    tile[e | ma] = y2;
+                    tile[e | ma | mb] = y3;
                 }
                 ++o;
             }
             __syncthreads();
         }
+        if (own) {
+            V r[EPT];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            uint32_t e = tid + (uint32_t)j * NT;
-            if (e < TS) s[base | (e & lowmask) | hoff[e >> L]] = tile[e];
+            for (int j = 0; j < EPT; ++j) r[j] = tile[tid + (uint32_t)j * NT];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) s[gidx(tid + (uint32_t)j * NT)] = r[j];
         }
         __syncthreads();
     }
@@ -460,8 +504,8 @@ static void launch_k(void* psi, const TilePass& p, const TileOp* ops, uint64_t n
         (void)hipFuncSetAttribute((const void*)k_tile<K, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds + 4096);
     });
-    hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(kTileThreads), lds, stream(), (V*)psi, p, ops, ninst,
-                       stride, ostride);
+    hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
+                       ninst, stride, ostride);
 }
 
 template <typename V, typename T>
@@ -550,8 +594,9 @@ int execute_fused(lq_state* st, const lq_gate* gates, int count) {
 
 using namespace lq;
 
-extern "C" int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits) {
-    clear_error();
+namespace lq {
+
+int set_fused_config_internal(int tile_bits_c128, int tile_bits_c64, int low_bits) {
     int t1 = tile_bits_c128 > 0 ? tile_bits_c128 : 12;
     int t2 = tile_bits_c64 > 0 ? tile_bits_c64 : 13;
     int lo = low_bits > 0 ? low_bits : 5;
@@ -564,6 +609,19 @@ extern "C" int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int lo
     g_tile_c64 = t2;
     g_low = lo;
     return LQ_OK;
+}
+
+void get_fused_config(int* tile_c128, int* tile_c64, int* low) {
+    *tile_c128 = g_tile_c128;
+    *tile_c64 = g_tile_c64;
+    *low = g_low;
+}
+
+}  // namespace lq
+
+extern "C" int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits) {
+    clear_error();
+    return set_fused_config_internal(tile_bits_c128, tile_bits_c64, low_bits);
 }
 
 extern "C" int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, int count, int tile_bits,

@@ -23,6 +23,38 @@ constexpr int kWave = 64;
 constexpr int kLaneBits = 6;
 constexpr int kThreads = 256;
 
+// Native two-element vectors for the nontemporal builtins.
+template <typename T> struct nvec2;
+template <> struct nvec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct nvec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+// Global load/store of one amplitude; NT selects the nontemporal (streaming)
+// form, which keeps a pass over a state far larger than L2 from evicting
+// useful lines.
+template <bool NT, typename V>
+__device__ __forceinline__ V gload(const V* p) {
+    using T = decltype(p->x);
+    using N = typename nvec2<T>::type;
+    N v;
+    if constexpr (NT) v = __builtin_nontemporal_load(reinterpret_cast<const N*>(p));
+    else v = *reinterpret_cast<const N*>(p);
+    V r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+}
+
+template <bool NT, typename V>
+__device__ __forceinline__ void gstore(V* p, V a) {
+    using T = decltype(a.x);
+    using N = typename nvec2<T>::type;
+    N v;
+    v.x = a.x;
+    v.y = a.y;
+    if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<N*>(p));
+    else *reinterpret_cast<N*>(p) = v;
+}
+
 template <typename T> struct ChunkBits {
     int pos[4];  // fixed positions in chunk-index space (amp bit - 6), ascending
     int count;
@@ -70,7 +102,7 @@ __device__ __forceinline__ void pair_op(const Mat2<T>& m, V a0, V a1, V& o0, V& 
 constexpr int kOpHadamard = 3;
 
 // Target bit >= 6: each work unit is a pair of chunks (target bit 0 / 1).
-template <typename V, typename T, int OP, int U>
+template <typename V, typename T, int OP, int U, bool NTM>
 __global__ void __launch_bounds__(kThreads) k_pair_hi(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
                                                       uint64_t chunk_or, uint64_t tstride, uint32_t lowctrl,
                                                       Mat2<T> m) {
@@ -87,8 +119,8 @@ __global__ void __launch_bounds__(kThreads) k_pair_hi(V* __restrict__ psi, uint6
         if (u < units) {
             uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
             idx[j] = (c << kLaneBits) | lane;
-            a0[j] = psi[idx[j]];
-            a1[j] = psi[idx[j] + tstride];
+            a0[j] = gload<NTM>(psi + idx[j]);
+            a1[j] = gload<NTM>(psi + idx[j] + tstride);
         }
     }
 #pragma unroll
@@ -96,14 +128,14 @@ __global__ void __launch_bounds__(kThreads) k_pair_hi(V* __restrict__ psi, uint6
         if (idx[j] != ~0ull && act) {
             V o0, o1;
             pair_op<OP, V, T>(m, a0[j], a1[j], o0, o1);
-            psi[idx[j]] = o0;
-            psi[idx[j] + tstride] = o1;
+            gstore<NTM>(psi + idx[j], o0);
+            gstore<NTM>(psi + idx[j] + tstride, o1);
         }
     }
 }
 
 // Target bit < 6: each work unit is one chunk; partner via lane shuffle.
-template <typename V, typename T, int OP, int U>
+template <typename V, typename T, int OP, int U, bool NTM>
 __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
                                                       uint64_t chunk_or, int tbit, uint32_t lowctrl,
                                                       uint32_t lane_limit, Mat2<T> m) {
@@ -125,7 +157,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint6
         if (u < units) {
             uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
             idx[j] = (c << kLaneBits) | lane;
-            if (in_range) a[j] = psi[idx[j]];
+            if (in_range) a[j] = gload<NTM>(psi + idx[j]);
         }
     }
 #pragma unroll
@@ -135,10 +167,10 @@ __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint6
             V o0, o1;
             if (hi_half) {
                 pair_op<OP, V, T>(m, p, a[j], o0, o1);
-                psi[idx[j]] = o1;
+                gstore<NTM>(psi + idx[j], o1);
             } else {
                 pair_op<OP, V, T>(m, a[j], p, o0, o1);
-                psi[idx[j]] = o0;
+                gstore<NTM>(psi + idx[j], o0);
             }
         }
     }
@@ -146,7 +178,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint6
 
 // Diagonal phases: every amplitude whose fixed bits are all 1 is multiplied
 // by d[bit(sel0) | bit(sel1) << 1].
-template <typename V, typename T, int U>
+template <typename V, typename T, int U, bool NTM>
 __global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
                                                    uint64_t chunk_or, uint32_t lowfixed, uint32_t lane_limit,
                                                    int sel0, int sel1, Diag4<T> d) {
@@ -163,7 +195,7 @@ __global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t
         if (u < units && act) {
             uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
             idx[j] = (c << kLaneBits) | lane;
-            a[j] = psi[idx[j]];
+            a[j] = gload<NTM>(psi + idx[j]);
         }
     }
 #pragma unroll
@@ -173,7 +205,9 @@ __global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t
             int k = 0;
             if (sel0 >= 0) k |= (int)((i >> sel0) & 1ull);
             if (sel1 >= 0) k |= (int)((i >> sel1) & 1ull) << 1;
-            psi[i] = cmul(d.d[2 * k], d.d[2 * k + 1], a[j]);
+            const T pr = k == 3 ? d.d[6] : k == 2 ? d.d[4] : k == 1 ? d.d[2] : d.d[0];
+            const T pi = k == 3 ? d.d[7] : k == 2 ? d.d[5] : k == 1 ? d.d[3] : d.d[1];
+            gstore<NTM>(psi + i, cmul(pr, pi, a[j]));
         }
     }
 }
@@ -243,7 +277,10 @@ __global__ void __launch_bounds__(kThreads) k_dense(const double2* __restrict__ 
 }
 
 // ---- host launchers -----------------------------------------------------------------
-constexpr int kUnroll = 4;
+// Variant knobs (autotuned): work units per wave (2, 4, 8) and nontemporal
+// global access.
+static int g_unroll = 4;
+static bool g_nontemporal = false;
 
 struct Split {
     ChunkBits<double> fb{};   // chunk-space fixed positions (positions only)
@@ -266,11 +303,39 @@ static Split split_bits(int n, uint64_t ones, uint64_t zeros) {
     return s;
 }
 
-static unsigned blocks_for_units(uint64_t units) {
-    uint64_t waves = (units + kUnroll - 1) / kUnroll;
+static unsigned blocks_for_units(uint64_t units, int unroll) {
+    uint64_t waves = (units + unroll - 1) / unroll;
     uint64_t blocks = (waves * kWave + kThreads - 1) / kThreads;
     if (blocks < 1) blocks = 1;
     return (unsigned)blocks;
+}
+
+template <typename V, typename T, int OP, int U, bool NTM>
+static void pair_go(V* psi, bool hi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, int tbit,
+                    uint32_t lowmask, uint32_t lane_limit, const Mat2<T>& m) {
+    unsigned g = blocks_for_units(units, U);
+    if (hi)
+        hipLaunchKernelGGL((k_pair_hi<V, T, OP, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb,
+                           chunk_or, 1ull << tbit, lowmask, m);
+    else
+        hipLaunchKernelGGL((k_pair_lo<V, T, OP, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb,
+                           chunk_or, tbit, lowmask, lane_limit, m);
+}
+
+template <typename V, typename T, int OP>
+static void pair_pick(V* psi, bool hi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, int tbit,
+                      uint32_t lowmask, uint32_t lane_limit, const Mat2<T>& m) {
+#define LQ_PAIR(U, NTM) pair_go<V, T, OP, U, NTM>(psi, hi, units, fb, chunk_or, tbit, lowmask, lane_limit, m)
+    if (g_nontemporal) {
+        if (g_unroll == 2) LQ_PAIR(2, true);
+        else if (g_unroll == 8) LQ_PAIR(8, true);
+        else LQ_PAIR(4, true);
+    } else {
+        if (g_unroll == 2) LQ_PAIR(2, false);
+        else if (g_unroll == 8) LQ_PAIR(8, false);
+        else LQ_PAIR(4, false);
+    }
+#undef LQ_PAIR
 }
 
 template <typename T, typename V>
@@ -279,58 +344,18 @@ static int launch_pair_t(lq_state* st, int tbit, uint64_t ctrl, const double md[
     for (int i = 0; i < 8; ++i) m.m[i] = (T)md[i];
     const int n = st->n;
     V* psi = (V*)st->d_amps;
-    if (tbit >= kLaneBits) {
-        Split s = split_bits(n, ctrl, 1ull << tbit);
-        ChunkBits<T> fb;
-        fb.count = s.fb.count;
-        for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
-        uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
-        unsigned g = blocks_for_units(units);
-        uint64_t ts = 1ull << tbit;
-        switch (op) {
-            case kOpMatrix:
-                hipLaunchKernelGGL((k_pair_hi<V, T, kOpMatrix, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, ts, s.lowmask, m);
-                break;
-            case kOpSwap:
-                hipLaunchKernelGGL((k_pair_hi<V, T, kOpSwap, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, ts, s.lowmask, m);
-                break;
-            case kOpNegY:
-                hipLaunchKernelGGL((k_pair_hi<V, T, kOpNegY, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, ts, s.lowmask, m);
-                break;
-            default:
-                hipLaunchKernelGGL((k_pair_hi<V, T, kOpHadamard, kUnroll>), dim3(g), dim3(kThreads), 0, stream(),
-                                   psi, units, fb, s.chunk_or, ts, s.lowmask, m);
-                break;
-        }
-    } else {
-        Split s = split_bits(n, ctrl, 0);
-        ChunkBits<T> fb;
-        fb.count = s.fb.count;
-        for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
-        uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
-        uint32_t lane_limit = n >= kLaneBits ? (uint32_t)kWave : (1u << n);
-        unsigned g = blocks_for_units(units);
-        switch (op) {
-            case kOpMatrix:
-                hipLaunchKernelGGL((k_pair_lo<V, T, kOpMatrix, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m);
-                break;
-            case kOpSwap:
-                hipLaunchKernelGGL((k_pair_lo<V, T, kOpSwap, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m);
-                break;
-            case kOpNegY:
-                hipLaunchKernelGGL((k_pair_lo<V, T, kOpNegY, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), psi,
-                                   units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m);
-                break;
-            default:
-                hipLaunchKernelGGL((k_pair_lo<V, T, kOpHadamard, kUnroll>), dim3(g), dim3(kThreads), 0, stream(),
-                                   psi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m);
-                break;
-        }
+    const bool hi = tbit >= kLaneBits;
+    Split s = split_bits(n, ctrl, hi ? (1ull << tbit) : 0);
+    ChunkBits<T> fb;
+    fb.count = s.fb.count;
+    for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
+    uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
+    uint32_t lane_limit = n >= kLaneBits ? (uint32_t)kWave : (1u << n);
+    switch (op) {
+        case kOpMatrix: pair_pick<V, T, kOpMatrix>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
+        case kOpSwap: pair_pick<V, T, kOpSwap>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
+        case kOpNegY: pair_pick<V, T, kOpNegY>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
+        default: pair_pick<V, T, kOpHadamard>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
     }
     LQ_LAUNCH_CHECK();
     return LQ_OK;
@@ -339,6 +364,14 @@ static int launch_pair_t(lq_state* st, int tbit, uint64_t ctrl, const double md[
 int launch_pair(lq_state* st, int tbit, uint64_t ctrl_mask, const double m[8], int op) {
     if (st->dtype == LQ_C128) return launch_pair_t<double, double2>(st, tbit, ctrl_mask, m, op);
     return launch_pair_t<float, float2>(st, tbit, ctrl_mask, m, op);
+}
+
+template <typename V, typename T, int U, bool NTM>
+static void diag_go(V* psi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, uint32_t lowmask,
+                    uint32_t lane_limit, int sel0, int sel1, const Diag4<T>& d) {
+    unsigned g = blocks_for_units(units, U);
+    hipLaunchKernelGGL((k_diag<V, T, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb, chunk_or,
+                       lowmask, lane_limit, sel0, sel1, d);
 }
 
 template <typename T, typename V>
@@ -351,9 +384,18 @@ static int launch_diag_t(lq_state* st, uint64_t fixed, int sel0, int sel1, const
     for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
     uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
     uint32_t lane_limit = st->n >= kLaneBits ? (uint32_t)kWave : (1u << st->n);
-    unsigned g = blocks_for_units(units);
-    hipLaunchKernelGGL((k_diag<V, T, kUnroll>), dim3(g), dim3(kThreads), 0, stream(), (V*)st->d_amps, units, fb,
-                       s.chunk_or, s.lowmask, lane_limit, sel0, sel1, d);
+    V* psi = (V*)st->d_amps;
+#define LQ_DIAG(U, NTM) diag_go<V, T, U, NTM>(psi, units, fb, s.chunk_or, s.lowmask, lane_limit, sel0, sel1, d)
+    if (g_nontemporal) {
+        if (g_unroll == 2) LQ_DIAG(2, true);
+        else if (g_unroll == 8) LQ_DIAG(8, true);
+        else LQ_DIAG(4, true);
+    } else {
+        if (g_unroll == 2) LQ_DIAG(2, false);
+        else if (g_unroll == 8) LQ_DIAG(8, false);
+        else LQ_DIAG(4, false);
+    }
+#undef LQ_DIAG
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
@@ -361,6 +403,16 @@ static int launch_diag_t(lq_state* st, uint64_t fixed, int sel0, int sel1, const
 int launch_diag(lq_state* st, uint64_t fixed_mask, int sel0, int sel1, const double d[8]) {
     if (st->dtype == LQ_C128) return launch_diag_t<double, double2>(st, fixed_mask, sel0, sel1, d);
     return launch_diag_t<float, float2>(st, fixed_mask, sel0, sel1, d);
+}
+
+int set_gate_kernel_config(int unroll, int nontemporal) {
+    if (unroll != 2 && unroll != 4 && unroll != 8) {
+        set_error("unroll must be 2, 4 or 8");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    g_unroll = unroll;
+    g_nontemporal = nontemporal != 0;
+    return LQ_OK;
 }
 
 static unsigned grid_stride_blocks(uint64_t count) {

@@ -40,12 +40,31 @@ def test_every_gate_every_target(hip, n):
     gates = all_gates(n, rng)
     if n > 6:  # keep the 3-qubit combinations bounded
         gates = [g for g in gates if g.kind != abi.TOFFOLI] + [
-            G(abi.TOFFOLI, [a, b, c]) for a, b, c in [(0, n - 1, 3), (n - 1, 0, n - 2), (2, 5, 0), (6, 1, n - 1)]]
+            G(abi.TOFFOLI, [a, b, c]) for a, b, c in [(0, n - 1, 3), (n - 1, 0, n - 2), (2, 5, 0), (n - 1, 1, 6 % (n - 1))]]
     for g in gates:
         got = gpu_apply(psi, n, [g])
         want = oracle.apply_gate(psi, n, g)
         err = np.max(np.abs(got - want))
         assert err <= AMP_TOL, (g.kind, list(g.qubits), err)
+
+
+@pytest.mark.parametrize("unroll,nt", [(2, 0), (4, 1), (8, 0), (8, 1)])
+def test_gate_kernel_variants(hip, unroll, nt):
+    """Every per-gate kernel variant the autotuner can pick gives the oracle's
+    result (complex128 and complex64)."""
+    say("variant unroll=%d nt=%d" % (unroll, nt))
+    lq._lib.call("lq_set_gate_kernel_config", unroll, nt)
+    try:
+        from tests.test_planner import random_circuit
+        for n in (5, 9, 14):
+            rng = np.random.default_rng(77 + n)
+            gates = random_circuit(n, 60, rng)
+            psi = bf.random_state(n, rng)
+            want = oracle.execute(psi, n, gates)
+            assert np.max(np.abs(gpu_apply(psi, n, gates) - want)) <= AMP_TOL
+            assert np.max(np.abs(gpu_apply(psi, n, gates, lq.C64) - want)) <= C64_TOL
+    finally:
+        lq._lib.call("lq_set_gate_kernel_config", 4, 0)
 
 
 @pytest.mark.parametrize("n", [7, 12])
