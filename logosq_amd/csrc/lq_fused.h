@@ -23,12 +23,24 @@ struct TileOp {
     int32_t kind;
     int32_t lb0, lb1;   // local (tile) bit positions
     int32_t sel0, sel1; // global bit positions for diagonal selectors (-1: none)
-    int32_t pad;
+    int8_t sp0, sp1;    // positions of lb0 / lb1 inside the op's group subcube
+    int8_t pad[2];
     uint64_t ctrl;      // global bits that must be 1 (controls, fixed bits)
     double m[32];       // pair: 8, quad: 32, diag: 8 (four phases)
 };
 
 constexpr int kMaxTileHigh = 28;
+constexpr int kGroupBits = 4;  // a group acts on at most 4 tile bits (16-amplitude subcubes)
+
+// A run of consecutive ops of one pass whose non-diagonal target bits all lie
+// in a set S of kGroupBits tile bits: each thread gathers one 2^4 subcube
+// (all values of the S bits) into registers and applies the whole run there.
+struct TileGroup {
+    int32_t op_first;   // relative to the pass's first op
+    int32_t op_count;
+    int32_t spos[kGroupBits];  // local positions of S, ascending
+    uint64_t gbit[kGroupBits]; // the corresponding global bits
+};
 
 struct TilePass {
     int32_t L;          // low contiguous bits 0..L-1 in every tile
@@ -36,6 +48,8 @@ struct TilePass {
     int32_t hpos[kMaxTileHigh];  // their global positions, ascending
     int32_t op_first;   // ops of this pass: [op_first, op_first + op_count)
     int32_t op_count;
+    int32_t group_first; // groups of this pass (grouped kernel, K >= 8)
+    int32_t group_count;
     uint64_t ntiles;    // 2^(n - L - hb)
 };
 
@@ -61,6 +75,7 @@ struct Plan {
     std::vector<Step> steps;
     std::vector<TileOp> ops;   // concatenated per pass
     std::vector<int> op_src;   // source gate of each op
+    std::vector<TileGroup> groups;
     int passes = 0;
 };
 
@@ -75,7 +90,7 @@ int default_low_bits(int n, int dtype);
 
 // Launch one tile pass over `ninst` states laid out `inst_stride` amplitudes
 // apart, with per-instance op tables `ops_stride` ops apart.
-int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
-                     uint64_t inst_stride, uint64_t ops_stride);
+int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, const TileGroup* d_groups,
+                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride);
 
 }  // namespace lq

@@ -224,11 +224,61 @@ static TileOp to_tile_op(const Prim& p, int L, const int* hpos, int hb) {
     return t;
 }
 
+// Partition a pass's ops into groups: maximal runs whose non-diagonal local
+// target bits fit kGroupBits bits (diagonal ops join any group).
+static void build_groups(Plan& plan, TilePass& pass, int K) {
+    pass.group_first = (int)plan.groups.size();
+    pass.group_count = 0;
+    if (K < 8) return;  // small tiles use the op-by-op kernel
+    auto need = [&](const TileOp& t) -> uint32_t {
+        if (t.kind == kTDiag) return 0u;
+        uint32_t m = 1u << t.lb0;
+        if (t.kind == kTSwap2 || t.kind == kTQuad) m |= 1u << t.lb1;
+        return m;
+    };
+    int i = 0;
+    while (i < pass.op_count) {
+        uint32_t S = 0;
+        int j = i;
+        while (j < pass.op_count) {
+            uint32_t nb = need(plan.ops[pass.op_first + j]);
+            if (__builtin_popcount(S | nb) > kGroupBits) break;
+            S |= nb;
+            ++j;
+        }
+        // pad with the highest free tile bits: lanes then walk the low bits
+        for (int b = K - 1; b >= 0 && __builtin_popcount(S) < kGroupBits; --b) S |= 1u << b;
+        TileGroup g;
+        std::memset(&g, 0, sizeof(g));
+        g.op_first = i;
+        g.op_count = j - i;
+        int k = 0;
+        for (int b = 0; b < K; ++b)
+            if (S & (1u << b)) {
+                g.spos[k] = b;
+                g.gbit[k] = 1ull << (b < pass.L ? b : pass.hpos[b - pass.L]);
+                ++k;
+            }
+        for (int o = i; o < j; ++o) {
+            TileOp& t = plan.ops[pass.op_first + o];
+            t.sp0 = t.sp1 = -1;
+            for (int q = 0; q < kGroupBits; ++q) {
+                if (g.spos[q] == t.lb0) t.sp0 = (int8_t)q;
+                if (g.spos[q] == t.lb1) t.sp1 = (int8_t)q;
+            }
+        }
+        plan.groups.push_back(g);
+        pass.group_count++;
+        i = j;
+    }
+}
+
 void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan) {
     plan.n = n;
     plan.steps.clear();
     plan.ops.clear();
     plan.op_src.clear();
+    plan.groups.clear();
     plan.passes = 0;
     K = std::min(K, n);
     if (n <= K) {
@@ -282,6 +332,7 @@ void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan
                 plan.ops.push_back(to_tile_op(prims[idx], L, st.pass.hpos, st.pass.hb));
                 plan.op_src.push_back(prims[idx].src);
             }
+            build_groups(plan, st.pass, K);
             plan.steps.push_back(st);
             plan.passes++;
             pending.swap(keep);
@@ -414,10 +465,14 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, T
                 const uint64_t ctrl = op[o].ctrl;
                 const T m0 = (T)op[o].m[0], m1 = (T)op[o].m[1], m2 = (T)op[o].m[2], m3 = (T)op[o].m[3];
                 const T m4 = (T)op[o].m[4], m5 = (T)op[o].m[5], m6 = (T)op[o].m[6], m7 = (T)op[o].m[7];
+                // TS/2 pairs: PPT per thread when TS/2 >= NT, else one pair
+                // for the first TS/2 threads
+                constexpr bool PFULL = TS / 2 >= (uint32_t)NT;
+                constexpr int PPT = PFULL ? (int)(TS / 2 / NT) : 1;
 #pragma unroll
-                for (int j = 0; j < (FULL ? EPT / 2 : 1); ++j) {
+                for (int j = 0; j < PPT; ++j) {
                     const uint32_t q = tid + (uint32_t)j * NT;
-                    if (!FULL && q >= TS / 2) continue;
+                    if (!PFULL && q >= TS / 2) continue;
                     const uint32_t e0 = ins0(q, lb);
                     const uint32_t e1 = e0 | (1u << lb);
                     if (ctrl && (gidx(e0) & ctrl) != ctrl) continue;
@@ -492,28 +547,267 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, T
 }
 
 // ---------------------------------------------------------------------------------
-// launcher
+// grouped kernel (tiles of >= 2^8 amplitudes)
 // ---------------------------------------------------------------------------------
-typedef void (*tile_kernel_t)(void*, TilePass, const TileOp*, uint64_t, uint64_t, uint64_t);
+// LDS slot of tile element e: XOR-swizzles the low 4 bits with bits 4..7 so
+// that 16-amplitude subcubes over any 4 bits, read by consecutive lanes, fall
+// in different LDS banks.
+__device__ __forceinline__ uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
+
+template <int P, typename T>
+__device__ __forceinline__ void g_pair(T* xr, T* xi, const TileOp& op, int kind, uint64_t gb, const uint64_t* go) {
+    const uint64_t ctrl = op.ctrl;
+    const T m0 = (T)op.m[0], m1 = (T)op.m[1], m2 = (T)op.m[2], m3 = (T)op.m[3];
+    const T m4 = (T)op.m[4], m5 = (T)op.m[5], m6 = (T)op.m[6], m7 = (T)op.m[7];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        if (s & (1 << P)) continue;
+        const int t = s | (1 << P);
+        const bool on = ((gb | go[s]) & ctrl) == ctrl;
+        const T ar = xr[s], ai = xi[s], br = xr[t], bi = xi[t];
+        T o0r, o0i, o1r, o1i;
+        if (kind == kTPairMat) {
+            o0r = (m0 * ar - m1 * ai) + (m2 * br - m3 * bi);
+            o0i = (m0 * ai + m1 * ar) + (m2 * bi + m3 * br);
+            o1r = (m4 * ar - m5 * ai) + (m6 * br - m7 * bi);
+            o1i = (m4 * ai + m5 * ar) + (m6 * bi + m7 * br);
+        } else if (kind == kTPairSwap) {
+            o0r = br;
+            o0i = bi;
+            o1r = ar;
+            o1i = ai;
+        } else if (kind == kTPairNegY) {
+            // a0' = i*a1, a1' = -i*a0 (single_qubit.rs:193-195)
+            o0r = (T)0 * br - (T)1 * bi;
+            o0i = (T)0 * bi + (T)1 * br;
+            o1r = (T)0 * ar + (T)1 * ai;
+            o1i = (T)0 * ai - (T)1 * ar;
+        } else {
+            // s*(a0 + a1), s*(a0 - a1) (single_qubit.rs:248-249)
+            const T sr = ar + br, si = ai + bi, dr = ar - br, di = ai - bi;
+            o0r = m0 * sr - (T)0 * si;
+            o0i = m0 * si + (T)0 * sr;
+            o1r = m0 * dr - (T)0 * di;
+            o1i = m0 * di + (T)0 * dr;
+        }
+        xr[s] = on ? o0r : ar;
+        xi[s] = on ? o0i : ai;
+        xr[t] = on ? o1r : br;
+        xi[t] = on ? o1i : bi;
+    }
+}
+
+// SWAP of subcube positions P0 (value 1) and P1 (value 0).
+template <int P0, int P1, typename T>
+__device__ __forceinline__ void g_swap2(T* xr, T* xi) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        if (!((s >> P0) & 1) || ((s >> P1) & 1)) continue;
+        const int t = s ^ (1 << P0) ^ (1 << P1);
+        T r = xr[s], i = xi[s];
+        xr[s] = xr[t];
+        xi[s] = xi[t];
+        xr[t] = r;
+        xi[t] = i;
+    }
+}
+
+// 4x4 block on subcube positions PA (bit a) and PB (bit b), sub-index
+// (bit a << 1) | bit b.
+template <int PA, int PB, typename T>
+__device__ __forceinline__ void g_quad(T* xr, T* xi, const double* m) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        if (((s >> PA) & 1) || ((s >> PB) & 1)) continue;
+        const int ix[4] = {s, s | (1 << PB), s | (1 << PA), s | (1 << PA) | (1 << PB)};
+        T ar[4], ai[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ar[c] = xr[ix[c]];
+            ai[c] = xi[ix[c]];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            T yr = (T)0, yi = (T)0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const T er = (T)m[8 * r + 2 * c], ei = (T)m[8 * r + 2 * c + 1];
+                yr += er * ar[c] - ei * ai[c];
+                yi += er * ai[c] + ei * ar[c];
+            }
+            xr[ix[r]] = yr;
+            xi[ix[r]] = yi;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t gb, const uint64_t* go) {
+    const int kind = op.kind;
+    if (kind == kTDiag) {
+        const uint64_t ctrl = op.ctrl;
+        const int s0 = op.sel0, s1 = op.sel1;
+        const T r0 = (T)op.m[0], i0 = (T)op.m[1], r1 = (T)op.m[2], i1 = (T)op.m[3];
+        const T r2 = (T)op.m[4], i2 = (T)op.m[5], r3 = (T)op.m[6], i3 = (T)op.m[7];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const uint64_t g = gb | go[s];
+            const bool on = (g & ctrl) == ctrl;
+            const bool b0 = s0 >= 0 && ((g >> s0) & 1ull);
+            const bool b1 = s1 >= 0 && ((g >> s1) & 1ull);
+            const T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
+            const T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
+            const T yr = pr * xr[s] - pi * xi[s];
+            const T yi = pr * xi[s] + pi * xr[s];
+            xr[s] = on ? yr : xr[s];
+            xi[s] = on ? yi : xi[s];
+        }
+    } else if (kind <= kTPairH) {
+        switch (op.sp0) {
+            case 0: g_pair<0, T>(xr, xi, op, kind, gb, go); break;
+            case 1: g_pair<1, T>(xr, xi, op, kind, gb, go); break;
+            case 2: g_pair<2, T>(xr, xi, op, kind, gb, go); break;
+            default: g_pair<3, T>(xr, xi, op, kind, gb, go); break;
+        }
+    } else {
+        const int code = op.sp0 * 4 + op.sp1;
+        if (kind == kTSwap2) {
+            switch (code) {
+#define LQ_SW(a, b) case a * 4 + b: g_swap2<a, b, T>(xr, xi); break;
+                LQ_SW(0, 1) LQ_SW(0, 2) LQ_SW(0, 3) LQ_SW(1, 0) LQ_SW(1, 2) LQ_SW(1, 3)
+                LQ_SW(2, 0) LQ_SW(2, 1) LQ_SW(2, 3) LQ_SW(3, 0) LQ_SW(3, 1) LQ_SW(3, 2)
+#undef LQ_SW
+                default: break;
+            }
+        } else {
+            switch (code) {
+#define LQ_QD(a, b) case a * 4 + b: g_quad<a, b, T>(xr, xi, op.m); break;
+                LQ_QD(0, 1) LQ_QD(0, 2) LQ_QD(0, 3) LQ_QD(1, 0) LQ_QD(1, 2) LQ_QD(1, 3)
+                LQ_QD(2, 0) LQ_QD(2, 1) LQ_QD(2, 3) LQ_QD(3, 0) LQ_QD(3, 1) LQ_QD(3, 2)
+#undef LQ_QD
+                default: break;
+            }
+        }
+    }
+}
 
 template <int K, typename V, typename T>
-static void launch_k(void* psi, const TilePass& p, const TileOp* ops, uint64_t ninst, uint64_t stride,
-                     uint64_t ostride, unsigned grid, size_t lds) {
+__global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, TilePass p,
+                                                          const TileOp* __restrict__ ops,
+                                                          const TileGroup* __restrict__ groups, uint64_t ninst,
+                                                          uint64_t inst_stride, uint64_t ops_stride) {
+    static_assert(K >= 8, "grouped tiles need at least 256 amplitudes");
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NT = tile_threads(K);
+    constexpr int EPT = (int)(TS / NT);
+    constexpr uint32_t NSUB = TS >> 4;
+    constexpr bool SFULL = NSUB >= (uint32_t)NT;
+    constexpr int SPT = SFULL ? (int)(NSUB / NT) : 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V* tile = reinterpret_cast<V*>(smem);
+    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem + TS * sizeof(V));
+    const int L = p.L;
+    const uint32_t lowmask = (1u << L) - 1u;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t h = tid; h < (1u << p.hb); h += NT) {
+        uint64_t o = 0;
+        for (int j = 0; j < p.hb; ++j)
+            if ((h >> j) & 1u) o |= 1ull << p.hpos[j];
+        hoff[h] = o;
+    }
+    __syncthreads();
+    const TileGroup* grp = groups + p.group_first;
+    const uint64_t total = ninst * p.ntiles;
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint64_t inst = w / p.ntiles;
+        const uint64_t t = w - inst * p.ntiles;
+        V* s = psi + inst * inst_stride;
+        const TileOp* op = ops + inst * ops_stride + p.op_first;
+        uint64_t base = t << L;
+        for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
+        auto gidx = [&](uint32_t e) -> uint64_t { return base | (e & lowmask) | hoff[e >> L]; };
+        {
+            V r[EPT];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) r[j] = s[gidx(tid + (uint32_t)j * NT)];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) tile[swz(tid + (uint32_t)j * NT)] = r[j];
+        }
+        __syncthreads();
+        for (int gi = 0; gi < p.group_count; ++gi) {
+            const TileGroup& G = grp[gi];
+            const int s0 = G.spos[0], s1 = G.spos[1], s2 = G.spos[2], s3 = G.spos[3];
+            uint64_t go[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                go[c] = ((c & 1) ? G.gbit[0] : 0ull) | ((c & 2) ? G.gbit[1] : 0ull) | ((c & 4) ? G.gbit[2] : 0ull) |
+                        ((c & 8) ? G.gbit[3] : 0ull);
+#pragma unroll
+            for (int k = 0; k < SPT; ++k) {
+                const uint32_t q = tid + (uint32_t)k * NT;
+                if (!SFULL && q >= NSUB) break;
+                const uint32_t eb = ins0(ins0(ins0(ins0(q, s0), s1), s2), s3);
+                const uint64_t gb = gidx(eb);
+                T xr[16], xi[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const uint32_t e = eb | ((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
+                                       ((uint32_t)((c >> 2) & 1) << s2) | ((uint32_t)((c >> 3) & 1) << s3);
+                    const V v = tile[swz(e)];
+                    xr[c] = v.x;
+                    xi[c] = v.y;
+                }
+                for (int o = G.op_first; o < G.op_first + G.op_count; ++o) g_apply<T>(xr, xi, op[o], gb, go);
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const uint32_t e = eb | ((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
+                                       ((uint32_t)((c >> 2) & 1) << s2) | ((uint32_t)((c >> 3) & 1) << s3);
+                    V v;
+                    v.x = xr[c];
+                    v.y = xi[c];
+                    tile[swz(e)] = v;
+                }
+            }
+            __syncthreads();
+        }
+        {
+            V r[EPT];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) r[j] = tile[swz(tid + (uint32_t)j * NT)];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) s[gidx(tid + (uint32_t)j * NT)] = r[j];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// launcher
+// ---------------------------------------------------------------------------------
+template <int K, typename V, typename T>
+static void launch_k(void* psi, const TilePass& p, const TileOp* ops, const TileGroup* groups, uint64_t ninst,
+                     uint64_t stride, uint64_t ostride, unsigned grid, size_t lds) {
     static std::once_flag once;
-    std::call_once(once, [&] {
-        (void)hipFuncSetAttribute((const void*)k_tile<K, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds + 4096);
-    });
-    hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
-                       ninst, stride, ostride);
+    if constexpr (K >= 8) {
+        std::call_once(once, [&] {
+            (void)hipFuncSetAttribute((const void*)k_tileg<K, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds + 4096);
+        });
+        hipLaunchKernelGGL((k_tileg<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
+                           groups, ninst, stride, ostride);
+    } else {
+        (void)groups;
+        hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
+                           ninst, stride, ostride);
+    }
 }
 
 template <typename V, typename T>
-static int dispatch_k(int K, void* psi, const TilePass& p, const TileOp* ops, uint64_t ninst, uint64_t stride,
-                      uint64_t ostride, unsigned grid, size_t lds) {
+static int dispatch_k(int K, void* psi, const TilePass& p, const TileOp* ops, const TileGroup* groups,
+                      uint64_t ninst, uint64_t stride, uint64_t ostride, unsigned grid, size_t lds) {
     switch (K) {
 #define LQ_K(k) \
-    case k: launch_k<k, V, T>(psi, p, ops, ninst, stride, ostride, grid, lds); break;
+    case k: launch_k<k, V, T>(psi, p, ops, groups, ninst, stride, ostride, grid, lds); break;
         LQ_K(1) LQ_K(2) LQ_K(3) LQ_K(4) LQ_K(5) LQ_K(6) LQ_K(7) LQ_K(8) LQ_K(9) LQ_K(10) LQ_K(11) LQ_K(12)
         LQ_K(13) LQ_K(14)
 #undef LQ_K
@@ -524,8 +818,8 @@ static int dispatch_k(int K, void* psi, const TilePass& p, const TileOp* ops, ui
     return LQ_OK;
 }
 
-int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
-                     uint64_t inst_stride, uint64_t ops_stride) {
+int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, const TileGroup* d_groups,
+                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride) {
     const int K = p.L + p.hb;
     const size_t amp = dtype == LQ_C128 ? 16 : 8;
     const size_t lds = (amp << K) + (sizeof(uint64_t) << p.hb);
@@ -533,45 +827,49 @@ int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_op
     unsigned grid = (unsigned)std::min<uint64_t>(total, 4096);
     int rc;
     if (dtype == LQ_C128)
-        rc = dispatch_k<double2, double>(K, psi, p, d_ops, ninst, inst_stride, ops_stride, grid, lds);
+        rc = dispatch_k<double2, double>(K, psi, p, d_ops, d_groups, ninst, inst_stride, ops_stride, grid, lds);
     else
-        rc = dispatch_k<float2, float>(K, psi, p, d_ops, ninst, inst_stride, ops_stride, grid, lds);
+        rc = dispatch_k<float2, float>(K, psi, p, d_ops, d_groups, ninst, inst_stride, ops_stride, grid, lds);
     if (rc) return rc;
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
 
-// Device copy of a plan's ops (grown on demand; reused across calls).
-static TileOp* g_ops_dev = nullptr;
-static size_t g_ops_cap = 0;
-static TileOp* g_ops_host = nullptr;
+// Device copy of a plan's ops and groups (grown on demand; reused across
+// calls): [ops ...][groups ...].
+static unsigned char* g_plan_dev = nullptr;
+static unsigned char* g_plan_host = nullptr;
+static size_t g_plan_cap = 0;
 
-static int upload_ops(const std::vector<TileOp>& ops, TileOp** out) {
-    size_t need = std::max<size_t>(ops.size(), 1);
-    if (need > g_ops_cap) {
-        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-        if (g_ops_dev) (void)hipFree(g_ops_dev);
-        if (g_ops_host) (void)hipHostFree(g_ops_host);
-        g_ops_dev = nullptr;
-        g_ops_host = nullptr;
-        size_t cap = std::max<size_t>(need, 1024);
-        LQ_HIP_CHECK(hipMalloc((void**)&g_ops_dev, cap * sizeof(TileOp)));
-        LQ_HIP_CHECK(hipHostMalloc((void**)&g_ops_host, cap * sizeof(TileOp), hipHostMallocDefault));
-        g_ops_cap = cap;
-    } else {
-        // the pinned staging buffer may still feed an earlier copy
-        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+int upload_plan(const std::vector<TileOp>& ops, const std::vector<TileGroup>& groups, TileOp** d_ops,
+                TileGroup** d_groups) {
+    size_t ob = ops.size() * sizeof(TileOp);
+    size_t gb = groups.size() * sizeof(TileGroup);
+    size_t need = ob + gb + 256;
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));  // the staging buffer may feed an earlier copy
+    if (need > g_plan_cap) {
+        if (g_plan_dev) (void)hipFree(g_plan_dev);
+        if (g_plan_host) (void)hipHostFree(g_plan_host);
+        g_plan_dev = nullptr;
+        g_plan_host = nullptr;
+        size_t cap = std::max<size_t>(need, 1 << 20);
+        LQ_HIP_CHECK(hipMalloc((void**)&g_plan_dev, cap));
+        LQ_HIP_CHECK(hipHostMalloc((void**)&g_plan_host, cap, hipHostMallocDefault));
+        g_plan_cap = cap;
     }
-    std::memcpy(g_ops_host, ops.data(), ops.size() * sizeof(TileOp));
-    LQ_HIP_CHECK(hipMemcpyAsync(g_ops_dev, g_ops_host, ops.size() * sizeof(TileOp), hipMemcpyHostToDevice, stream()));
-    *out = g_ops_dev;
+    size_t goff = (ob + 255) & ~(size_t)255;
+    if (ob) std::memcpy(g_plan_host, ops.data(), ob);
+    if (gb) std::memcpy(g_plan_host + goff, groups.data(), gb);
+    LQ_HIP_CHECK(hipMemcpyAsync(g_plan_dev, g_plan_host, goff + gb, hipMemcpyHostToDevice, stream()));
+    *d_ops = (TileOp*)g_plan_dev;
+    *d_groups = (TileGroup*)(g_plan_dev + goff);
     return LQ_OK;
 }
 
-int run_plan(lq_state* st, const Plan& plan, const TileOp* d_ops) {
+int run_plan(lq_state* st, const Plan& plan, const TileOp* d_ops, const TileGroup* d_groups) {
     for (const Step& s : plan.steps) {
         int rc = s.normalize ? normalize_device(st, nullptr)
-                             : launch_tile_pass(st->d_amps, st->dtype, s.pass, d_ops, 1, 0, 0);
+                             : launch_tile_pass(st->d_amps, st->dtype, s.pass, d_ops, d_groups, 1, 0, 0);
         if (rc) return rc;
     }
     return LQ_OK;
@@ -585,9 +883,10 @@ int execute_fused(lq_state* st, const lq_gate* gates, int count) {
     plan_passes(st->n, prims, default_tile_bits(st->n, st->dtype), default_low_bits(st->n, st->dtype), plan);
     if (plan.ops.empty() && plan.steps.empty()) return LQ_OK;
     TileOp* d_ops = nullptr;
-    rc = upload_ops(plan.ops, &d_ops);
+    TileGroup* d_groups = nullptr;
+    rc = upload_plan(plan.ops, plan.groups, &d_ops, &d_groups);
     if (rc) return rc;
-    return run_plan(st, plan, d_ops);
+    return run_plan(st, plan, d_ops, d_groups);
 }
 
 }  // namespace lq
