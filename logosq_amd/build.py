@@ -27,9 +27,15 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
+# Per-file flags.  lq_fused.hip: LLVM's common-code sinking would merge the
+# per-position copies of the register subcube code into one block with a
+# computed array index, which sends the subcube to scratch memory.
+EXTRA = {"lq_fused.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *HIP_FLAGS, *EXTRA.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))

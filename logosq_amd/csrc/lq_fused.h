@@ -23,11 +23,20 @@ struct TileOp {
     int32_t kind;
     int32_t lb0, lb1;   // local (tile) bit positions
     int32_t sel0, sel1; // global bit positions for diagonal selectors (-1: none)
-    int8_t sp0, sp1;    // positions of lb0 / lb1 inside the op's group subcube
-    int8_t pad[2];
+    int32_t osel0, osel1; // grouped kernel: selectors outside the group's subcube (-1: none / inside)
+    // 32-bit fields throughout: the kernels read them with scalar loads
+    int32_t sp0, sp1;   // positions of lb0 / lb1 inside the op's group subcube
+    int32_t s0p, s1p;   // grouped kernel: subcube positions of selectors inside S (-1: none)
+    int32_t cin;        // grouped kernel: controls inside S, as a mask of subcube positions
+    int32_t pad;
     uint64_t ctrl;      // global bits that must be 1 (controls, fixed bits)
+    uint64_t cout;      // grouped kernel: the controls outside S
     double m[32];       // pair: 8, quad: 32, diag: 8 (four phases)
 };
+
+// Specialised pair kinds used by the grouped kernel (classified at plan time).
+constexpr int32_t kTPairReal = 7;  // all four entries real (RY, H-free real rotations)
+constexpr int32_t kTPairRX = 8;    // real diagonal, imaginary off-diagonal (RX)
 
 constexpr int kMaxTileHigh = 28;
 constexpr int kGroupBits = 4;  // a group acts on at most 4 tile bits (16-amplitude subcubes)
@@ -50,6 +59,7 @@ struct TilePass {
     int32_t op_count;
     int32_t group_first; // groups of this pass (grouped kernel, K >= 8)
     int32_t group_count;
+    int32_t rich;       // the pass holds SWAP / 4x4 ops (kernel variant with those paths)
     uint64_t ntiles;    // 2^(n - L - hb)
 };
 

@@ -225,11 +225,23 @@ static TileOp to_tile_op(const Prim& p, int L, const int* hpos, int hb) {
 }
 
 // Partition a pass's ops into groups: maximal runs whose non-diagonal local
-// target bits fit kGroupBits bits (diagonal ops join any group).
+// target bits fit kGroupBits bits (diagonal ops join any group).  Free
+// subcube positions go to the bits the group's ops test (controls,
+// selectors), so those tests become compile-time per subcube slot; then to
+// the highest free tile bits (lanes walk the low bits).  Each op gets its
+// in-subcube / out-of-subcube control and selector split, and 2x2 matrices
+// are classified (real, RX-like, general) for the leanest arithmetic.
 static void build_groups(Plan& plan, TilePass& pass, int K) {
     pass.group_first = (int)plan.groups.size();
     pass.group_count = 0;
+    pass.rich = 0;
     if (K < 8) return;  // small tiles use the op-by-op kernel
+    auto local_of = [&](int g) -> int {
+        if (g < pass.L) return g;
+        for (int j = 0; j < pass.hb; ++j)
+            if (pass.hpos[j] == g) return pass.L + j;
+        return -1;
+    };
     auto need = [&](const TileOp& t) -> uint32_t {
         if (t.kind == kTDiag) return 0u;
         uint32_t m = 1u << t.lb0;
@@ -246,26 +258,74 @@ static void build_groups(Plan& plan, TilePass& pass, int K) {
             S |= nb;
             ++j;
         }
-        // pad with the highest free tile bits: lanes then walk the low bits
+        // local bits tested by the group's ops, by frequency
+        int freq[32] = {0};
+        for (int o = i; o < j; ++o) {
+            const TileOp& t = plan.ops[pass.op_first + o];
+            for (int b = 0; b < 64; ++b)
+                if (t.ctrl & (1ull << b)) {
+                    int lb = local_of(b);
+                    if (lb >= 0) freq[lb]++;
+                }
+            for (int sb : {t.sel0, t.sel1})
+                if (sb >= 0) {
+                    int lb = local_of(sb);
+                    if (lb >= 0) freq[lb]++;
+                }
+        }
+        while (__builtin_popcount(S) < kGroupBits) {
+            int best = -1;
+            for (int b = K - 1; b >= 0; --b)
+                if (!(S & (1u << b)) && freq[b] > 0 && (best < 0 || freq[b] > freq[best])) best = b;
+            if (best < 0) break;
+            S |= 1u << best;
+        }
         for (int b = K - 1; b >= 0 && __builtin_popcount(S) < kGroupBits; --b) S |= 1u << b;
         TileGroup g;
         std::memset(&g, 0, sizeof(g));
         g.op_first = i;
         g.op_count = j - i;
         int k = 0;
+        uint64_t Sg = 0;
+        int spos_of[32];
+        for (int b = 0; b < 32; ++b) spos_of[b] = -1;
         for (int b = 0; b < K; ++b)
             if (S & (1u << b)) {
                 g.spos[k] = b;
                 g.gbit[k] = 1ull << (b < pass.L ? b : pass.hpos[b - pass.L]);
+                Sg |= g.gbit[k];
+                spos_of[b] = k;
                 ++k;
             }
         for (int o = i; o < j; ++o) {
             TileOp& t = plan.ops[pass.op_first + o];
-            t.sp0 = t.sp1 = -1;
-            for (int q = 0; q < kGroupBits; ++q) {
-                if (g.spos[q] == t.lb0) t.sp0 = (int8_t)q;
-                if (g.spos[q] == t.lb1) t.sp1 = (int8_t)q;
+            t.sp0 = t.lb0 >= 0 ? spos_of[t.lb0] : -1;
+            t.sp1 = t.lb1 >= 0 ? spos_of[t.lb1] : -1;
+            t.cout = t.ctrl & ~Sg;
+            t.cin = 0;
+            for (int q = 0; q < kGroupBits; ++q)
+                if (t.ctrl & g.gbit[q]) t.cin |= 1 << q;
+            t.s0p = t.s1p = -1;
+            t.osel0 = t.osel1 = -1;
+            if (t.sel0 >= 0) {
+                int lb = local_of(t.sel0);
+                if (lb >= 0 && spos_of[lb] >= 0) t.s0p = spos_of[lb];
+                else t.osel0 = t.sel0;
             }
+            if (t.sel1 >= 0) {
+                int lb = local_of(t.sel1);
+                if (lb >= 0 && spos_of[lb] >= 0) t.s1p = spos_of[lb];
+                else t.osel1 = t.sel1;
+            }
+            if (t.kind == kTPairMat) {
+                const double* m = t.m;
+                if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) t.kind = kTPairReal;
+                else if (m[1] == 0.0 && m[7] == 0.0 && m[2] == 0.0 && m[4] == 0.0) t.kind = kTPairRX;
+            }
+        }
+        for (int o = i; o < j; ++o) {
+            const int kd = plan.ops[pass.op_first + o].kind;
+            if (kd == kTSwap2 || kd == kTQuad) pass.rich = 1;
         }
         plan.groups.push_back(g);
         pass.group_count++;
@@ -399,10 +459,12 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, T
     __syncthreads();
     const uint64_t total = ninst * p.ntiles;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
-        const uint64_t inst = w / p.ntiles;
-        const uint64_t t = w - inst * p.ntiles;
-        V* s = psi + inst * inst_stride;
-        const TileOp* op = ops + inst * ops_stride + p.op_first;
+        // wave-uniform by construction; readfirstlane lets the compiler see it,
+        // so op parameters are fetched with scalar loads
+        const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));
+        const uint64_t t = w - (uint64_t)inst * p.ntiles;
+        V* s = psi + (uint64_t)inst * inst_stride;
+        const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;
         uint64_t base = t << L;
         for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
         // global index of element e of this tile
@@ -554,46 +616,63 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, T
 // in different LDS banks.
 __device__ __forceinline__ uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
 
-template <int P, typename T>
-__device__ __forceinline__ void g_pair(T* xr, T* xi, const TileOp& op, int kind, uint64_t gb, const uint64_t* go) {
-    const uint64_t ctrl = op.ctrl;
+// One pair op on subcube position P.  `cin` lists the controls inside the
+// subcube: for a compile-time slot s the test is uniform (a scalar branch),
+// never a per-element select.
+template <int P, int KIND, typename T>
+__device__ __forceinline__ void g_pair(T* xr, T* xi, const TileOp& op, int cin) {
     const T m0 = (T)op.m[0], m1 = (T)op.m[1], m2 = (T)op.m[2], m3 = (T)op.m[3];
     const T m4 = (T)op.m[4], m5 = (T)op.m[5], m6 = (T)op.m[6], m7 = (T)op.m[7];
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         if (s & (1 << P)) continue;
+        if ((s & cin) != cin) continue;
         const int t = s | (1 << P);
-        const bool on = ((gb | go[s]) & ctrl) == ctrl;
         const T ar = xr[s], ai = xi[s], br = xr[t], bi = xi[t];
-        T o0r, o0i, o1r, o1i;
-        if (kind == kTPairMat) {
-            o0r = (m0 * ar - m1 * ai) + (m2 * br - m3 * bi);
-            o0i = (m0 * ai + m1 * ar) + (m2 * bi + m3 * br);
-            o1r = (m4 * ar - m5 * ai) + (m6 * br - m7 * bi);
-            o1i = (m4 * ai + m5 * ar) + (m6 * bi + m7 * br);
-        } else if (kind == kTPairSwap) {
-            o0r = br;
-            o0i = bi;
-            o1r = ar;
-            o1i = ai;
-        } else if (kind == kTPairNegY) {
+        if (KIND == kTPairReal) {
+            xr[s] = m0 * ar + m2 * br;
+            xi[s] = m0 * ai + m2 * bi;
+            xr[t] = m4 * ar + m6 * br;
+            xi[t] = m4 * ai + m6 * bi;
+        } else if (KIND == kTPairRX) {
+            // [[c, i*m3], [i*m5, c']]: re/im of c*a + i*m3*b etc.
+            xr[s] = m0 * ar - m3 * bi;
+            xi[s] = m0 * ai + m3 * br;
+            xr[t] = m6 * br - m5 * ai;
+            xi[t] = m6 * bi + m5 * ar;
+        } else if (KIND == kTPairMat) {
+            xr[s] = (m0 * ar - m1 * ai) + (m2 * br - m3 * bi);
+            xi[s] = (m0 * ai + m1 * ar) + (m2 * bi + m3 * br);
+            xr[t] = (m4 * ar - m5 * ai) + (m6 * br - m7 * bi);
+            xi[t] = (m4 * ai + m5 * ar) + (m6 * bi + m7 * br);
+        } else if (KIND == kTPairSwap) {
+            xr[s] = br;
+            xi[s] = bi;
+            xr[t] = ar;
+            xi[t] = ai;
+        } else if (KIND == kTPairNegY) {
             // a0' = i*a1, a1' = -i*a0 (single_qubit.rs:193-195)
-            o0r = (T)0 * br - (T)1 * bi;
-            o0i = (T)0 * bi + (T)1 * br;
-            o1r = (T)0 * ar + (T)1 * ai;
-            o1i = (T)0 * ai - (T)1 * ar;
+            xr[s] = -bi;
+            xi[s] = br;
+            xr[t] = ai;
+            xi[t] = -ar;
         } else {
             // s*(a0 + a1), s*(a0 - a1) (single_qubit.rs:248-249)
-            const T sr = ar + br, si = ai + bi, dr = ar - br, di = ai - bi;
-            o0r = m0 * sr - (T)0 * si;
-            o0i = m0 * si + (T)0 * sr;
-            o1r = m0 * dr - (T)0 * di;
-            o1i = m0 * di + (T)0 * dr;
+            xr[s] = m0 * (ar + br);
+            xi[s] = m0 * (ai + bi);
+            xr[t] = m0 * (ar - br);
+            xi[t] = m0 * (ai - bi);
         }
-        xr[s] = on ? o0r : ar;
-        xi[s] = on ? o0i : ai;
-        xr[t] = on ? o1r : br;
-        xi[t] = on ? o1i : bi;
+    }
+}
+
+template <int KIND, typename T>
+__device__ __forceinline__ void g_pair_at(T* xr, T* xi, const TileOp& op, int cin) {
+    switch (op.sp0) {
+        case 0: g_pair<0, KIND, T>(xr, xi, op, cin); break;
+        case 1: g_pair<1, KIND, T>(xr, xi, op, cin); break;
+        case 2: g_pair<2, KIND, T>(xr, xi, op, cin); break;
+        default: g_pair<3, KIND, T>(xr, xi, op, cin); break;
     }
 }
 
@@ -641,35 +720,43 @@ __device__ __forceinline__ void g_quad(T* xr, T* xi, const double* m) {
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t gb, const uint64_t* go) {
+template <bool RICH, typename T>
+__device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t gb) {
+    // controls outside the subcube are fixed for the whole subcube
+    if ((gb & op.cout) != op.cout) return;
     const int kind = op.kind;
+    const int cin = op.cin;
     if (kind == kTDiag) {
-        const uint64_t ctrl = op.ctrl;
-        const int s0 = op.sel0, s1 = op.sel1;
+        const int s0p = op.s0p, s1p = op.s1p;
+        const int tb0 = op.osel0 >= 0 ? (int)((gb >> op.osel0) & 1ull) : 0;
+        const int tb1 = op.osel1 >= 0 ? (int)((gb >> op.osel1) & 1ull) : 0;
         const T r0 = (T)op.m[0], i0 = (T)op.m[1], r1 = (T)op.m[2], i1 = (T)op.m[3];
         const T r2 = (T)op.m[4], i2 = (T)op.m[5], r3 = (T)op.m[6], i3 = (T)op.m[7];
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            const uint64_t g = gb | go[s];
-            const bool on = (g & ctrl) == ctrl;
-            const bool b0 = s0 >= 0 && ((g >> s0) & 1ull);
-            const bool b1 = s1 >= 0 && ((g >> s1) & 1ull);
+            if ((s & cin) != cin) continue;
+            const int b0 = s0p >= 0 ? ((s >> s0p) & 1) : tb0;
+            const int b1 = s1p >= 0 ? ((s >> s1p) & 1) : tb1;
             const T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
             const T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
             const T yr = pr * xr[s] - pi * xi[s];
             const T yi = pr * xi[s] + pi * xr[s];
-            xr[s] = on ? yr : xr[s];
-            xi[s] = on ? yi : xi[s];
+            xr[s] = yr;
+            xi[s] = yi;
         }
-    } else if (kind <= kTPairH) {
-        switch (op.sp0) {
-            case 0: g_pair<0, T>(xr, xi, op, kind, gb, go); break;
-            case 1: g_pair<1, T>(xr, xi, op, kind, gb, go); break;
-            case 2: g_pair<2, T>(xr, xi, op, kind, gb, go); break;
-            default: g_pair<3, T>(xr, xi, op, kind, gb, go); break;
-        }
-    } else {
+    } else if (kind == kTPairReal) {
+        g_pair_at<kTPairReal, T>(xr, xi, op, cin);
+    } else if (kind == kTPairRX) {
+        g_pair_at<kTPairRX, T>(xr, xi, op, cin);
+    } else if (kind == kTPairMat) {
+        g_pair_at<kTPairMat, T>(xr, xi, op, cin);
+    } else if (kind == kTPairSwap) {
+        g_pair_at<kTPairSwap, T>(xr, xi, op, cin);
+    } else if (kind == kTPairNegY) {
+        g_pair_at<kTPairNegY, T>(xr, xi, op, cin);
+    } else if (kind == kTPairH) {
+        g_pair_at<kTPairH, T>(xr, xi, op, cin);
+    } else if constexpr (RICH) {
         const int code = op.sp0 * 4 + op.sp1;
         if (kind == kTSwap2) {
             switch (code) {
@@ -691,8 +778,10 @@ __device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t
     }
 }
 
-template <int K, typename V, typename T>
-__global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, TilePass p,
+// At least two waves per SIMD: two 64 KiB tiles per CU are resident, so the
+// register budget is 256 VGPRs per lane.
+template <int K, bool RICH, typename V, typename T>
+__global__ void __launch_bounds__(tile_threads(K), 2) k_tileg(V* __restrict__ psi, TilePass p,
                                                           const TileOp* __restrict__ ops,
                                                           const TileGroup* __restrict__ groups, uint64_t ninst,
                                                           uint64_t inst_stride, uint64_t ops_stride) {
@@ -719,10 +808,12 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, 
     const TileGroup* grp = groups + p.group_first;
     const uint64_t total = ninst * p.ntiles;
     for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
-        const uint64_t inst = w / p.ntiles;
-        const uint64_t t = w - inst * p.ntiles;
-        V* s = psi + inst * inst_stride;
-        const TileOp* op = ops + inst * ops_stride + p.op_first;
+        // wave-uniform by construction; readfirstlane lets the compiler see it,
+        // so op parameters are fetched with scalar loads
+        const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));
+        const uint64_t t = w - (uint64_t)inst * p.ntiles;
+        V* s = psi + (uint64_t)inst * inst_stride;
+        const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;
         uint64_t base = t << L;
         for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
         auto gidx = [&](uint32_t e) -> uint64_t { return base | (e & lowmask) | hoff[e >> L]; };
@@ -737,11 +828,6 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, 
         for (int gi = 0; gi < p.group_count; ++gi) {
             const TileGroup& G = grp[gi];
             const int s0 = G.spos[0], s1 = G.spos[1], s2 = G.spos[2], s3 = G.spos[3];
-            uint64_t go[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-                go[c] = ((c & 1) ? G.gbit[0] : 0ull) | ((c & 2) ? G.gbit[1] : 0ull) | ((c & 4) ? G.gbit[2] : 0ull) |
-                        ((c & 8) ? G.gbit[3] : 0ull);
 #pragma unroll
             for (int k = 0; k < SPT; ++k) {
                 const uint32_t q = tid + (uint32_t)k * NT;
@@ -757,7 +843,7 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, 
                     xr[c] = v.x;
                     xi[c] = v.y;
                 }
-                for (int o = G.op_first; o < G.op_first + G.op_count; ++o) g_apply<T>(xr, xi, op[o], gb, go);
+                for (int o = G.op_first; o < G.op_first + G.op_count; ++o) g_apply<RICH, T>(xr, xi, op[o], gb);
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const uint32_t e = eb | ((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
@@ -784,17 +870,24 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tileg(V* __restrict__ psi, 
 // ---------------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------------
+template <int K, bool RICH, typename V, typename T>
+static void launch_g(void* psi, const TilePass& p, const TileOp* ops, const TileGroup* groups, uint64_t ninst,
+                     uint64_t stride, uint64_t ostride, unsigned grid, size_t lds) {
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)k_tileg<K, RICH, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds + 4096);
+    });
+    hipLaunchKernelGGL((k_tileg<K, RICH, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
+                       groups, ninst, stride, ostride);
+}
+
 template <int K, typename V, typename T>
 static void launch_k(void* psi, const TilePass& p, const TileOp* ops, const TileGroup* groups, uint64_t ninst,
                      uint64_t stride, uint64_t ostride, unsigned grid, size_t lds) {
-    static std::once_flag once;
     if constexpr (K >= 8) {
-        std::call_once(once, [&] {
-            (void)hipFuncSetAttribute((const void*)k_tileg<K, V, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds + 4096);
-        });
-        hipLaunchKernelGGL((k_tileg<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,
-                           groups, ninst, stride, ostride);
+        if (p.rich) launch_g<K, true, V, T>(psi, p, ops, groups, ninst, stride, ostride, grid, lds);
+        else launch_g<K, false, V, T>(psi, p, ops, groups, ninst, stride, ostride, grid, lds);
     } else {
         (void)groups;
         hipLaunchKernelGGL((k_tile<K, V, T>), dim3(grid), dim3(tile_threads(K)), lds, stream(), (V*)psi, p, ops,

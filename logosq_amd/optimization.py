@@ -451,7 +451,7 @@ class GradientDescent:
         return self
 
     def minimize(self, f, grad_f, initial_params):
-        x = np.ascontiguousarray(initial_params, dtype=np.float64)
+        x = np.array(initial_params, dtype=np.float64, copy=True)  # the caller's slice is not modified
         hist = np.zeros(max(1, self.max_iterations))
         res = LqOptResult()
         fo, go = _wrap(f, grad_f)
@@ -471,7 +471,7 @@ class Adam:
         return self
 
     def minimize(self, f, grad_f, initial_params):
-        x = np.ascontiguousarray(initial_params, dtype=np.float64)
+        x = np.array(initial_params, dtype=np.float64, copy=True)  # the caller's slice is not modified
         hist = np.zeros(max(1, self.max_iterations))
         res = LqOptResult()
         fo, go = _wrap(f, grad_f)
@@ -501,7 +501,7 @@ class VQE:
     def run(self, initial_parameters):
         arr, ng = _template_args(self.ansatz)
         codes, coeffs = _obs_args(self.hamiltonian)
-        x = np.ascontiguousarray(initial_parameters, dtype=np.float64)
+        x = np.array(initial_parameters, dtype=np.float64, copy=True)
         opt = self.optimizer
         hist = np.zeros(max(1, opt.max_iterations))
         res = LqOptResult()
