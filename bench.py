@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--cpu-gates", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the gradient / uneven-batch workloads")
     return ap.parse_args()
 
 
@@ -82,6 +83,75 @@ def num_passes(lq, n, dtype, descs):
     lq._lib.call("lq_plan_fused", n, dtype, abi.gate_array(descs), len(descs), 0, 0, order, None,
                  ctypes.byref(npass))
     return npass.value
+
+
+def ps_gradient_bench(lq, reps=3, cpu=True):
+    """All 2P+1 parameter-shift circuits of a 16-qubit, L=6 ring ansatz
+    (P=192) with the 48-term XYZ Heisenberg ring: batched (one launch per
+    pass for all 385 circuits) vs the plain sequential GPU method."""
+    import numpy as np
+    from logosq_amd import abi, algorithms
+    tmpl, params, codes, coeffs, j = algorithms.ps_gradient_workload(16, 6, 3)
+    arr = abi.gate_array(tmpl)
+    P = len(params)
+    out = {}
+    for name, mode, dtype in (("batched_c128", abi.GRAD_BATCHED, 0), ("batched_c64", abi.GRAD_BATCHED, 1),
+                              ("sequential_c128", abi.GRAD_SEQUENTIAL, 0)):
+        g = np.zeros(P)
+        e = ctypes.c_double()
+        ts = []
+        for _ in range(reps if mode == abi.GRAD_BATCHED else 1):
+            t0 = time.perf_counter()
+            lq._lib.call("lq_param_shift_gradient", 16, dtype, arr, len(tmpl),
+                         params.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), P, math.pi / 2,
+                         codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                         coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(coeffs), 0, mode,
+                         ctypes.byref(e), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"s_per_gradient": min(ts), "energy": e.value, "grad": g.copy()}
+    ref = out["sequential_c128"]["grad"]
+    res = {"workload": "PS gradient, 16 qubits, L=6 RY/RZ+CZ ring, P=192, XYZ ring (48 terms), 385 circuits"}
+    for k, v in out.items():
+        res[k] = {"ms_per_gradient": round(v["s_per_gradient"] * 1e3, 3),
+                  "max_abs_grad_diff_vs_sequential_c128": float(np.max(np.abs(v["grad"] - ref)))}
+    res["speedup_batched_vs_sequential"] = round(out["sequential_c128"]["s_per_gradient"] /
+                                                 out["batched_c128"]["s_per_gradient"], 2)
+    if cpu:
+        import oracle
+        k = 4  # gradient components timed on the CPU oracle (2k+1 circuits), scaled to 2P+1
+        t0 = time.perf_counter()
+        e0 = oracle.energy(16, tmpl, params, codes, coeffs)
+        for i in range(k):
+            pp = params.copy()
+            pp[i] += math.pi / 2
+            oracle.energy(16, tmpl, pp, codes, coeffs)
+            pp[i] -= math.pi
+            oracle.energy(16, tmpl, pp, codes, coeffs)
+        dt = time.perf_counter() - t0
+        res["cpu_oracle_s_per_gradient_extrapolated"] = round(dt / (2 * k + 1) * (2 * P + 1), 2)
+        res["cpu_sample"] = "%d of 385 circuits timed, single thread, scaled" % (2 * k + 1)
+        res["energy_vs_oracle"] = abs(e0 - out["sequential_c128"]["energy"])
+    return res
+
+
+def uneven_batch_bench(lq, reps=2):
+    """512 circuits, n ~ U{10..22}, L=4 ring ansatz, random 32-term Pauli sums:
+    complex64 and complex128 batches; per-circuit error of c64 vs c128."""
+    import numpy as np
+    from logosq_amd import algorithms
+    items = algorithms.uneven_batch_workload(512, 10, 22, 4, 32, 4)
+    res = {"workload": "512 circuits, n~U{10..22}, L=4 RY/RZ+CZ ring, 32-term random Pauli sums"}
+    energies = {}
+    for name, dtype in (("c64", 1), ("c128", 0)):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            energies[name] = lq.batch_expectations(items, dtype)
+            ts.append(time.perf_counter() - t0)
+        res[name] = {"s_per_batch": round(min(ts), 4), "circuits_per_s": round(len(items) / min(ts), 1)}
+    d = np.abs(energies["c64"] - energies["c128"])
+    res["c64_vs_c128_abs_err"] = {"max": float(d.max()), "mean": float(d.mean())}
+    return res
 
 
 def cpu_baseline(nq, ngates, seed, depth):
@@ -182,6 +252,13 @@ def main():
         bpl = 2 * c * N
         kname = "k_pair (RX on each of the %d targets)" % a.qubits
     achieved = bpl / avg_s / 1e9
+    extra = {}
+    if not a.no_extra:
+        st = None  # release the 2^30 state before the batch workloads
+        print("bench: gradient workload", file=sys.stderr, flush=True)
+        extra["ps_gradient_16q"] = ps_gradient_bench(lq, cpu=not a.no_cpu)
+        print("bench: uneven batch workload", file=sys.stderr, flush=True)
+        extra["uneven_batch"] = uneven_batch_bench(lq)
     cpu = None
     if not a.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -211,6 +288,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "%s, %d B per launch, avg %.3f ms" % (kname, bpl, avg_s * 1e3)},
         "cpu_baseline": cpu,
+        "workloads": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -226,6 +226,18 @@ int lq_param_shift_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int 
                             int num_params, double shift, const uint8_t* paulis, const double* coeffs,
                             int num_terms, int convention, int mode, double* energy_out, double* grad_out);
 
+/* ---- Batches (no reference counterpart: the reference evaluates circuits
+ * one after another) ------------------------------------------------------ */
+/* count circuits, circuit c on num_qubits[c] (<= 30) qubits with gates
+ * circuits[c][0 .. num_gates[c]) (angles from theta[]), each run from
+ * |0...0> in dtype and measured with its own Pauli sum (paulis[c]: a
+ * num_terms[c] x num_qubits[c] code matrix, coeffs[c]).  Circuits of one
+ * structure share one pass plan and run together, one launch per pass;
+ * every expectation is reduced in f64.  energies[c] receives the result. */
+int lq_batch_expectations(int count, const int* num_qubits, const lq_gate* const* circuits, const int* num_gates,
+                          const uint8_t* const* paulis, const double* const* coeffs, const int* num_terms,
+                          int convention, int dtype, double* energies);
+
 /* ---- Optimizers and VQE (src/optimization/optimizer.rs, vqe.rs) -------- */
 typedef double (*lq_objective_fn)(const double* x, int dim, void* user);
 typedef void (*lq_gradient_fn)(const double* x, int dim, double* grad, void* user);

@@ -15,5 +15,6 @@ from .optimization import (Pauli, PauliTerm, PauliObservable, Observable, Ansatz
                            EntanglingPattern, ParameterShift, FiniteDifference, GradientDescent, Adam, VQE,
                            heisenberg_ring, ry_rz_cz_ring_ansatz)
 from . import algorithms
+from .batch import batch_expectations
 
 __version__ = "0.1.0"

@@ -85,3 +85,78 @@ def brickwork_circuit(num_qubits, depth, seed):
         for q in range(layer % 2, num_qubits - 1, 2):
             c.cnot(q, q + 1)
     return c
+
+
+def ry_rz_ring_template(num_qubits, layers):
+    """Gate template of the benchmark ansatz: per layer RY(q), RZ(q) on every
+    qubit then CZ(q, q+1 mod n); parameter order layer -> qubit -> (RY, RZ)."""
+    from . import abi
+    t, k = [], 0
+    for _ in range(layers):
+        for q in range(num_qubits):
+            t.append(abi.make_gate(abi.RY, [q], [0.0], [k]))
+            t.append(abi.make_gate(abi.RZ, [q], [0.0], [k + 1]))
+            k += 2
+        for q in range(num_qubits):
+            t.append(abi.make_gate(abi.CZ, [q, (q + 1) % num_qubits]))
+    return t, k
+
+
+def bind(template, params):
+    """Descriptors with param slots replaced by the given angles."""
+    from . import abi
+    out = []
+    for g in template:
+        h = abi.LqGate()
+        ctypes_copy(h, g)
+        for k in range(3):
+            if h.param[k] >= 0:
+                h.theta[k] = float(params[h.param[k]])
+                h.param[k] = -1
+        out.append(h)
+    return out
+
+
+def ctypes_copy(dst, src):
+    import ctypes
+    ctypes.memmove(ctypes.addressof(dst), ctypes.addressof(src), ctypes.sizeof(src))
+
+
+def heisenberg_codes(num_qubits, jx, jy, jz):
+    """Pauli codes / coefficients of the periodic XYZ ring (3n terms)."""
+    codes, coeffs = [], []
+    for i in range(num_qubits):
+        j = (i + 1) % num_qubits
+        for p, c in ((1, jx), (2, jy), (3, jz)):
+            row = [0] * num_qubits
+            row[i] = row[j] = p
+            codes.append(row)
+            coeffs.append(c)
+    return np.array(codes, dtype=np.uint8), np.array(coeffs, dtype=np.float64)
+
+
+def ps_gradient_workload(num_qubits=16, layers=6, seed=3):
+    """Batched parameter-shift workload: couplings (Jx, Jy, Jz) ~ U[-1,1]^3
+    shared by every bond, params ~ U[0, 2pi)^(2nL)."""
+    r = SplitMix64(seed)
+    j = [2.0 * r.uniform() - 1.0 for _ in range(3)]
+    tmpl, P = ry_rz_ring_template(num_qubits, layers)
+    params = np.array([2.0 * math.pi * r.uniform() for _ in range(P)])
+    codes, coeffs = heisenberg_codes(num_qubits, *j)
+    return tmpl, params, codes, coeffs, j
+
+
+def uneven_batch_workload(count=512, nmin=10, nmax=22, layers=4, terms=32, seed=4):
+    """Uneven batch: circuit c has n_c ~ U{nmin..nmax} qubits, the ring ansatz
+    with L layers and params ~ U[0, 2pi), and a random `terms`-term Pauli sum
+    (each qubit's Pauli uniform over I, X, Y, Z; coefficients N(0,1))."""
+    r = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = nmin + min(nmax - nmin, int(r.uniform() * (nmax - nmin + 1)))
+        tmpl, P = ry_rz_ring_template(n, layers)
+        params = [2.0 * math.pi * r.uniform() for _ in range(P)]
+        codes = np.array([[min(3, int(r.uniform() * 4)) for _ in range(n)] for _ in range(terms)], dtype=np.uint8)
+        coeffs = np.array([r.normal() for _ in range(terms)])
+        out.append((n, bind(tmpl, params), codes, coeffs))
+    return out

@@ -1,0 +1,502 @@
+// Batched execution: many states of one size run the same pass plan with
+// per-instance op tables (their own angles), one launch per pass for the
+// whole batch; their Pauli-sum expectations are reduced by one work-item
+// kernel over every (instance, flip-mask group).  Used for
+//   - all 2P+1 parameter-shift circuits of a gradient (the reference runs them
+//     one after another: src/optimization/gradient.rs:53-74), and
+//   - batches of unrelated circuits of uneven qubit counts.
+// Accumulation is f64 for both amplitude types, in a fixed order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "lq_fused.h"
+
+namespace lq {
+
+int upload_plan(const std::vector<TileOp>& ops, const std::vector<TileGroup>& groups, TileOp** d_ops,
+                TileGroup** d_groups);
+void bind_template(const lq_gate* tmpl, int num_gates, const double* params, int num_params,
+                   std::vector<lq_gate>& out);
+
+// ---------------------------------------------------------------------------------
+// device buffers reused across calls
+// ---------------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int get(size_t bytes) {
+        if (bytes <= cap) return LQ_OK;
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return LQ_ERR_OUT_OF_MEMORY;
+        }
+        cap = bytes;
+        return LQ_OK;
+    }
+};
+
+struct HostBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int get(size_t bytes) {
+        if (bytes <= cap) return LQ_OK;
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        LQ_HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        cap = bytes;
+        return LQ_OK;
+    }
+};
+
+static DevBuf g_states, g_bops, g_items, g_terms, g_part, g_out;
+static HostBuf g_hops, g_hitems, g_hterms, g_hout;
+
+// ---------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------
+template <typename V>
+__global__ void k_fill_zero_states(V* __restrict__ psi, uint64_t dim, uint64_t total) {
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        V x;
+        x.x = (i & (dim - 1)) == 0 ? 1 : 0;
+        x.y = 0;
+        psi[i] = x;
+    }
+}
+
+// One work item: a flip-mask group of one instance's Pauli sum.
+struct ExpItem {
+    uint64_t state_off;  // amplitude offset of the instance's state
+    uint64_t x;          // flip mask; 0 = diagonal group
+    uint64_t work;       // amplitudes (diagonal) or pairs (flip) to visit
+    int32_t h;           // top bit of x
+    int32_t term_first;
+    int32_t term_count;
+    int32_t block_first; // first block of this item in the launch
+    int32_t nblocks;
+    int32_t inst;
+};
+
+constexpr int kEThreads = 256;
+
+template <int NT>
+__device__ __forceinline__ double bsum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict__ psi,
+                                                            const ExpItem* __restrict__ items, int nitems,
+                                                            const PauliTermDev* __restrict__ terms,
+                                                            double* __restrict__ partial) {
+    __shared__ double sh[kEThreads];
+    // locate the item of this block (items sorted by block_first)
+    int lo = 0, hi = nitems - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block_first <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const ExpItem it = items[lo];
+    const int lb = b - it.block_first;
+    const V* s = psi + it.state_off;
+    const PauliTermDev* tm = terms + it.term_first;
+    const uint64_t stride = (uint64_t)it.nblocks * kEThreads;
+    double acc = 0.0;
+    if (it.x == 0) {
+        for (uint64_t i = (uint64_t)lb * kEThreads + threadIdx.x; i < it.work; i += stride) {
+            V a = s[i];
+            double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+            double w = 0.0;
+            for (int t = 0; t < it.term_count; ++t) {
+                double c = tm[t].cr;
+                w += (__popcll(i & tm[t].zy) & 1) ? -c : c;
+            }
+            acc += p * w;
+        }
+    } else {
+        for (uint64_t k = (uint64_t)lb * kEThreads + threadIdx.x; k < it.work; k += stride) {
+            uint64_t i = insert_zero(k, it.h);
+            uint64_t j = i ^ it.x;
+            V a = s[i], c2 = s[j];
+            double zr = (double)a.x * (double)c2.x + (double)a.y * (double)c2.y;
+            double zi = (double)a.x * (double)c2.y - (double)a.y * (double)c2.x;
+            double wir = 0.0, wii = 0.0, wjr = 0.0, wji = 0.0;
+            for (int t = 0; t < it.term_count; ++t) {
+                PauliTermDev q = tm[t];
+                bool si = __popcll(i & q.zy) & 1;
+                bool sj = __popcll(j & q.zy) & 1;
+                wir += si ? -q.cr : q.cr;
+                wii += si ? -q.ci : q.ci;
+                wjr += sj ? -q.cr : q.cr;
+                wji += sj ? -q.ci : q.ci;
+            }
+            acc += zr * wir - zi * wii + zr * wjr + zi * wji;
+        }
+    }
+    double t = bsum<kEThreads>(acc, sh);
+    if (threadIdx.x == 0) partial[b] = t;
+}
+
+// energies[inst] = sum of the partials of the instance's blocks, in order.
+__global__ void __launch_bounds__(256) k_sum_ranges(const double* __restrict__ partial,
+                                                   const int* __restrict__ first, const int* __restrict__ count,
+                                                   double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int inst = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count[inst]; i += 256) s += partial[first[inst] + i];
+    double t = bsum<256>(s, sh);
+    if (threadIdx.x == 0) out[inst] = t;
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+// Per-instance Pauli sums: term tables concatenated, groups per instance.
+struct BatchObs {
+    const PauliSum* ps;  // instance's compiled sum
+    int term_base;       // offset of its terms in the concatenated table
+};
+
+static int blocks_for(uint64_t work) {
+    uint64_t b = (work + kEThreads * 8 - 1) / (kEThreads * 8);
+    if (b < 1) b = 1;
+    if (b > 1024) b = 1024;
+    return (int)b;
+}
+
+// Expectations of ninst states (dim amplitudes each, contiguous from psi).
+// obs_of[i] indexes `sums` (instances may share one sum).
+static int batch_expect(const void* psi, int dtype, uint64_t dim, int ninst, const std::vector<PauliSum>& sums,
+                        const std::vector<int>& obs_of, double* energies) {
+    // terms
+    std::vector<int> term_base(sums.size());
+    size_t nterms = 0;
+    for (size_t k = 0; k < sums.size(); ++k) {
+        term_base[k] = (int)nterms;
+        nterms += sums[k].terms.size();
+    }
+    // items
+    std::vector<ExpItem> items;
+    std::vector<int> first(ninst), count(ninst);
+    int blocks = 0;
+    for (int i = 0; i < ninst; ++i) {
+        const PauliSum& ps = sums[obs_of[i]];
+        first[i] = blocks;
+        for (const PauliGroup& g : ps.groups) {
+            ExpItem it;
+            std::memset(&it, 0, sizeof(it));
+            it.state_off = (uint64_t)i * dim;
+            it.x = g.x;
+            it.work = g.x ? dim / 2 : dim;
+            it.h = g.x ? 63 - __builtin_clzll(g.x) : -1;
+            it.term_first = term_base[obs_of[i]] + g.first;
+            it.term_count = g.count;
+            it.block_first = blocks;
+            it.nblocks = blocks_for(it.work);
+            it.inst = i;
+            blocks += it.nblocks;
+            items.push_back(it);
+        }
+        count[i] = blocks - first[i];
+    }
+    if (blocks == 0) {
+        for (int i = 0; i < ninst; ++i) energies[i] = 0.0;
+        return LQ_OK;
+    }
+    size_t ib = items.size() * sizeof(ExpItem);
+    size_t tb = std::max<size_t>(nterms, 1) * sizeof(PauliTermDev);
+    size_t rb = (size_t)ninst * 2 * sizeof(int);
+    int rc = g_items.get(ib + rb + 256);
+    if (!rc) rc = g_terms.get(tb);
+    if (!rc) rc = g_part.get((size_t)blocks * sizeof(double));
+    if (!rc) rc = g_out.get((size_t)ninst * sizeof(double));
+    if (!rc) rc = g_hitems.get(ib + rb + 256);
+    if (!rc) rc = g_hterms.get(tb);
+    if (!rc) rc = g_hout.get((size_t)ninst * sizeof(double));
+    if (rc) return rc;
+    size_t roff = (ib + 255) & ~(size_t)255;
+    char* hi = (char*)g_hitems.p;
+    std::memcpy(hi, items.data(), ib);
+    std::memcpy(hi + roff, first.data(), ninst * sizeof(int));
+    std::memcpy(hi + roff + ninst * sizeof(int), count.data(), ninst * sizeof(int));
+    PauliTermDev* ht = (PauliTermDev*)g_hterms.p;
+    for (size_t k = 0; k < sums.size(); ++k)
+        if (!sums[k].terms.empty())
+            std::memcpy(ht + term_base[k], sums[k].terms.data(), sums[k].terms.size() * sizeof(PauliTermDev));
+    LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, hi, roff + rb, hipMemcpyHostToDevice, stream()));
+    LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, ht, nterms * sizeof(PauliTermDev), hipMemcpyHostToDevice, stream()));
+    const int* d_first = (const int*)((char*)g_items.p + roff);
+    const int* d_count = d_first + ninst;
+    if (dtype == LQ_C128)
+        hipLaunchKernelGGL(k_expect_items<double2>, dim3(blocks), dim3(kEThreads), 0, stream(), (const double2*)psi,
+                           (const ExpItem*)g_items.p, (int)items.size(), (const PauliTermDev*)g_terms.p,
+                           (double*)g_part.p);
+    else
+        hipLaunchKernelGGL(k_expect_items<float2>, dim3(blocks), dim3(kEThreads), 0, stream(), (const float2*)psi,
+                           (const ExpItem*)g_items.p, (int)items.size(), (const PauliTermDev*)g_terms.p,
+                           (double*)g_part.p);
+    LQ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sum_ranges, dim3(ninst), dim3(256), 0, stream(), (const double*)g_part.p, d_first, d_count,
+                       (double*)g_out.p);
+    LQ_LAUNCH_CHECK();
+    LQ_HIP_CHECK(hipMemcpyAsync(g_hout.p, g_out.p, ninst * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+    std::memcpy(energies, g_hout.p, ninst * sizeof(double));
+    return LQ_OK;
+}
+
+// Run `ninst` circuits that share one structure (`plan`, built from
+// instance 0) from |0...0>, instance i with op table inst_ops[i].
+static int batch_run(int n, int dtype, const Plan& plan, const std::vector<TileOp>& inst_ops, int ninst,
+                     void* psi) {
+    const uint64_t dim = 1ull << n;
+    const size_t amp = dtype == LQ_C128 ? 16 : 8;
+    (void)amp;
+    uint64_t total = dim * (uint64_t)ninst;
+    unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+    if (dtype == LQ_C128)
+        hipLaunchKernelGGL(k_fill_zero_states<double2>, dim3(g), dim3(256), 0, stream(), (double2*)psi, dim, total);
+    else
+        hipLaunchKernelGGL(k_fill_zero_states<float2>, dim3(g), dim3(256), 0, stream(), (float2*)psi, dim, total);
+    LQ_LAUNCH_CHECK();
+    TileOp* d_ops = nullptr;
+    TileGroup* d_groups = nullptr;
+    int rc = upload_plan(inst_ops, plan.groups, &d_ops, &d_groups);
+    if (rc) return rc;
+    const uint64_t ostride = plan.ops.size();
+    for (const Step& s : plan.steps) {
+        if (s.normalize) {
+            // MatrixGate renormalisation: per instance
+            for (int i = 0; i < ninst; ++i) {
+                lq_state tmp;
+                tmp.n = n;
+                tmp.dtype = dtype;
+                tmp.dim = dim;
+                tmp.amp_bytes = dtype == LQ_C128 ? 16 : 8;
+                tmp.d_amps = (char*)psi + (size_t)i * dim * tmp.amp_bytes;
+                rc = normalize_device(&tmp, nullptr);
+                if (rc) return rc;
+            }
+            continue;
+        }
+        rc = launch_tile_pass(psi, dtype, s.pass, d_ops, d_groups, (uint64_t)ninst, dim, ostride);
+        if (rc) return rc;
+    }
+    return LQ_OK;
+}
+
+// Per-instance ops: the shared plan's ops with each instance's matrices.
+// prims_i must have the same structure as the prims the plan was built from.
+static void instance_ops(const Plan& plan, const std::vector<Prim>& prims_i, const std::vector<int>& first_prim,
+                         TileOp* out) {
+    for (size_t o = 0; o < plan.ops.size(); ++o) {
+        TileOp t = plan.ops[o];
+        const Prim& p = prims_i[first_prim[plan.op_src[o]]];
+        std::memcpy(t.m, p.m, sizeof(t.m));
+        out[o] = t;
+    }
+}
+
+static std::vector<int> first_prims(const std::vector<Prim>& prims, int num_gates) {
+    std::vector<int> f(num_gates, -1);
+    for (size_t k = 0; k < prims.size(); ++k)
+        if (prims[k].kind >= 0 && f[prims[k].src] < 0) f[prims[k].src] = (int)k;
+    return f;
+}
+
+// Instances per chunk so that chunk states fit a budget of device memory.
+static int chunk_instances(uint64_t dim, size_t amp, int total) {
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
+    size_t budget = std::min<size_t>(freeb / 2, (size_t)96 << 30);
+    size_t per = (size_t)dim * amp;
+    int k = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
+    return std::min(k, total);
+}
+
+int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
+                        double shift, const PauliSum& ps, double* energy_out, double* grad_out) {
+    const uint64_t dim = 1ull << n;
+    const size_t amp = dtype == LQ_C128 ? 16 : 8;
+    const int ninst = 2 * num_params + 1;
+    // structure from the base circuit
+    std::vector<lq_gate> bound;
+    bind_template(tmpl, num_gates, params, num_params, bound);
+    std::vector<Prim> base;
+    int rc = gates_to_prims(n, bound.data(), num_gates, base);
+    if (rc) return rc;
+    Plan plan;
+    plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+    std::vector<int> fp = first_prims(base, num_gates);
+    // which gates use each parameter slot
+    std::vector<std::vector<int>> users(num_params);
+    for (int g = 0; g < num_gates; ++g)
+        for (int k = 0; k < 3; ++k)
+            if (tmpl[g].param[k] >= 0 && tmpl[g].param[k] < num_params) {
+                auto& u = users[tmpl[g].param[k]];
+                if (u.empty() || u.back() != g) u.push_back(g);
+            }
+    const size_t nops = plan.ops.size();
+    std::vector<double> energies(ninst);
+    const int chunk = chunk_instances(dim, amp, ninst);
+    std::vector<TileOp> ops;
+    std::vector<PauliSum> sums(1, ps);
+    std::vector<double> p(params, params + num_params);
+    for (int c0 = 0; c0 < ninst; c0 += chunk) {
+        const int cn = std::min(chunk, ninst - c0);
+        ops.resize(nops * (size_t)cn);
+        for (int k = 0; k < cn; ++k) {
+            const int inst = c0 + k;
+            TileOp* o = ops.data() + nops * (size_t)k;
+            std::memcpy(o, plan.ops.data(), nops * sizeof(TileOp));
+            if (inst == 0) continue;
+            // instance 2i+1: theta_i + s, instance 2i+2: theta_i - s
+            const int i = (inst - 1) / 2;
+            const double sh = (inst % 2 == 1) ? shift : -shift;
+            p[i] = params[i] + sh;
+            for (int g : users[i]) {
+                lq_gate gg = tmpl[g];
+                for (int kk = 0; kk < 3; ++kk)
+                    if (gg.param[kk] >= 0) gg.theta[kk] = p[gg.param[kk]];
+                std::vector<Prim> one;
+                rc = gates_to_prims(n, &gg, 1, one);
+                if (rc) return rc;
+                for (size_t oo = 0; oo < nops; ++oo)
+                    if (plan.op_src[oo] == g) std::memcpy(o[oo].m, one[0].m, sizeof(o[oo].m));
+            }
+            p[i] = params[i];
+        }
+        rc = g_states.get((size_t)cn * dim * amp);
+        if (rc) return rc;
+        rc = batch_run(n, dtype, plan, ops, cn, g_states.p);
+        if (rc) return rc;
+        std::vector<int> obs_of(cn, 0);
+        rc = batch_expect(g_states.p, dtype, dim, cn, sums, obs_of, energies.data() + c0);
+        if (rc) return rc;
+    }
+    (void)fp;
+    if (energy_out) *energy_out = energies[0];
+    for (int i = 0; i < num_params; ++i) grad_out[i] = (energies[2 * i + 1] - energies[2 * i + 2]) / 2.0;
+    return LQ_OK;
+}
+
+// Structure key of a circuit: kinds, qubits, flags (angles excluded).
+static std::vector<int64_t> structure_key(int n, const lq_gate* g, int count) {
+    std::vector<int64_t> k;
+    k.reserve(2 + (size_t)count * 5);
+    k.push_back(n);
+    k.push_back(count);
+    for (int i = 0; i < count; ++i) {
+        k.push_back(g[i].kind);
+        k.push_back(g[i].qubits[0]);
+        k.push_back(g[i].qubits[1]);
+        k.push_back(g[i].qubits[2]);
+        k.push_back(g[i].flags);
+        if (g[i].kind == LQ_GATE_MATRIX1 || g[i].kind == LQ_GATE_MATRIX2) k.push_back(-1000 - i);  // no sharing
+    }
+    return k;
+}
+
+}  // namespace lq
+
+using namespace lq;
+
+extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_gate* const* circuits,
+                                     const int* num_gates, const uint8_t* const* paulis,
+                                     const double* const* coeffs, const int* num_terms, int convention, int dtype,
+                                     double* energies) {
+    clear_error();
+    if (count < 0 || (count > 0 && (!num_qubits || !circuits || !num_gates || !paulis || !coeffs || !num_terms ||
+                                    !energies))) {
+        set_error("invalid batch arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    if (dtype != LQ_C128 && dtype != LQ_C64) {
+        set_error("unknown dtype %d", dtype);
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    int rc = ensure_init();
+    if (rc) return rc;
+    // validate and compile observables
+    std::vector<PauliSum> sums(count);
+    for (int c = 0; c < count; ++c) {
+        if (num_qubits[c] < 1 || num_qubits[c] > 30 || num_gates[c] < 0 || (num_gates[c] > 0 && !circuits[c])) {
+            set_error("circuit %d: invalid size or gate list", c);
+            return LQ_ERR_INVALID_ARGUMENT;
+        }
+        for (int g = 0; g < num_gates[c]; ++g) {
+            rc = validate_gate(num_qubits[c], &circuits[c][g]);
+            if (rc) return rc;
+        }
+        rc = compile_pauli_sum(num_qubits[c], paulis[c], coeffs[c], num_terms[c], convention, &sums[c]);
+        if (rc) return rc;
+    }
+    // group circuits of one structure; largest states first
+    std::map<std::vector<int64_t>, std::vector<int>> groups;
+    for (int c = 0; c < count; ++c) groups[structure_key(num_qubits[c], circuits[c], num_gates[c])].push_back(c);
+    std::vector<std::vector<int>> order;
+    for (auto& kv : groups) order.push_back(kv.second);
+    std::sort(order.begin(), order.end(),
+              [&](const std::vector<int>& a, const std::vector<int>& b) { return num_qubits[a[0]] > num_qubits[b[0]]; });
+    const size_t amp = dtype == LQ_C128 ? 16 : 8;
+    for (const auto& members : order) {
+        const int n = num_qubits[members[0]];
+        const int ng = num_gates[members[0]];
+        const uint64_t dim = 1ull << n;
+        std::vector<Prim> base;
+        rc = gates_to_prims(n, circuits[members[0]], ng, base);
+        if (rc) return rc;
+        Plan plan;
+        plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+        std::vector<int> fp = first_prims(base, ng);
+        const size_t nops = plan.ops.size();
+        const int total = (int)members.size();
+        const int chunk = chunk_instances(dim, amp, total);
+        std::vector<TileOp> ops;
+        for (int c0 = 0; c0 < total; c0 += chunk) {
+            const int cn = std::min(chunk, total - c0);
+            ops.resize(nops * (size_t)cn);
+            std::vector<PauliSum> csums;
+            std::vector<int> obs_of(cn);
+            for (int k = 0; k < cn; ++k) {
+                const int c = members[c0 + k];
+                std::vector<Prim> pr;
+                rc = gates_to_prims(n, circuits[c], ng, pr);
+                if (rc) return rc;
+                instance_ops(plan, pr, fp, ops.data() + nops * (size_t)k);
+                csums.push_back(sums[c]);
+                obs_of[k] = k;
+            }
+            rc = g_states.get((size_t)cn * dim * amp);
+            if (rc) return rc;
+            rc = batch_run(n, dtype, plan, ops, cn, g_states.p);
+            if (rc) return rc;
+            std::vector<double> e(cn);
+            rc = batch_expect(g_states.p, dtype, dim, cn, csums, obs_of, e.data());
+            if (rc) return rc;
+            for (int k = 0; k < cn; ++k) energies[members[c0 + k]] = e[k];
+        }
+    }
+    return LQ_OK;
+}

@@ -28,7 +28,8 @@ struct TileOp {
     int32_t sp0, sp1;   // positions of lb0 / lb1 inside the op's group subcube
     int32_t s0p, s1p;   // grouped kernel: subcube positions of selectors inside S (-1: none)
     int32_t cin;        // grouped kernel: controls inside S, as a mask of subcube positions
-    int32_t pad;
+    int32_t mclass;     // pair ops: kTPairReal / kTPairRX when the gate kind guarantees the structure
+
     uint64_t ctrl;      // global bits that must be 1 (controls, fixed bits)
     uint64_t cout;      // grouped kernel: the controls outside S
     double m[32];       // pair: 8, quad: 32, diag: 8 (four phases)
@@ -73,6 +74,7 @@ struct Prim {
     uint64_t nd;        // bits acted on non-diagonally
     uint64_t ds;        // bits the op depends on diagonally (controls, selectors)
     int32_t src;        // index of the gate this primitive came from
+    int32_t mclass;     // kTPairReal for RY/CRY, kTPairRX for RX/CRX, else kTPairMat
 };
 
 struct Step {

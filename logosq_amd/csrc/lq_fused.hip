@@ -41,6 +41,7 @@ static Prim make_prim(int kind) {
     p.kind = kind;
     p.b0 = p.b1 = -1;
     p.sel0 = p.sel1 = -1;
+    p.mclass = kTPairMat;
     return p;
 }
 
@@ -106,7 +107,11 @@ int gates_to_prims(int n, const lq_gate* gates, int count, std::vector<Prim>& ou
             case LQ_GATE_CRY: {
                 bool ctl = g->kind == LQ_GATE_CRX || g->kind == LQ_GATE_CRY;
                 gate_pair_matrix(g->kind, th, m);
-                out.push_back(pair_prim(kTPairMat, bpos(n, ctl ? q[1] : q[0]), ctl ? bmask(n, q[0]) : 0, m));
+                Prim pp = pair_prim(kTPairMat, bpos(n, ctl ? q[1] : q[0]), ctl ? bmask(n, q[0]) : 0, m);
+                // the structure follows from the kind, for every angle
+                if (g->kind == LQ_GATE_RY || g->kind == LQ_GATE_CRY) pp.mclass = kTPairReal;
+                if (g->kind == LQ_GATE_RX || g->kind == LQ_GATE_CRX) pp.mclass = kTPairRX;
+                out.push_back(pp);
                 break;
             }
             case LQ_GATE_RZ:
@@ -220,6 +225,7 @@ static TileOp to_tile_op(const Prim& p, int L, const int* hpos, int hb) {
     t.sel0 = p.sel0;
     t.sel1 = p.sel1;
     t.ctrl = p.ctrl;
+    t.mclass = p.mclass;
     std::memcpy(t.m, p.m, sizeof(t.m));
     return t;
 }
@@ -317,11 +323,7 @@ static void build_groups(Plan& plan, TilePass& pass, int K) {
                 if (lb >= 0 && spos_of[lb] >= 0) t.s1p = spos_of[lb];
                 else t.osel1 = t.sel1;
             }
-            if (t.kind == kTPairMat) {
-                const double* m = t.m;
-                if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0) t.kind = kTPairReal;
-                else if (m[1] == 0.0 && m[7] == 0.0 && m[2] == 0.0 && m[4] == 0.0) t.kind = kTPairRX;
-            }
+            if (t.kind == kTPairMat) t.kind = t.mclass;
         }
         for (int o = i; o < j; ++o) {
             const int kd = plan.ops[pass.op_first + o].kind;
@@ -732,17 +734,46 @@ __device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t
         const int tb1 = op.osel1 >= 0 ? (int)((gb >> op.osel1) & 1ull) : 0;
         const T r0 = (T)op.m[0], i0 = (T)op.m[1], r1 = (T)op.m[2], i1 = (T)op.m[3];
         const T r2 = (T)op.m[4], i2 = (T)op.m[5], r3 = (T)op.m[6], i3 = (T)op.m[7];
+        if (s0p < 0 && s1p < 0) {
+            // selectors outside the subcube: one phase for the thread
+            const T pr = tb1 ? (tb0 ? r3 : r2) : (tb0 ? r1 : r0);
+            const T pi = tb1 ? (tb0 ? i3 : i2) : (tb0 ? i1 : i0);
 #pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            if ((s & cin) != cin) continue;
-            const int b0 = s0p >= 0 ? ((s >> s0p) & 1) : tb0;
-            const int b1 = s1p >= 0 ? ((s >> s1p) & 1) : tb1;
-            const T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
-            const T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
-            const T yr = pr * xr[s] - pi * xi[s];
-            const T yi = pr * xi[s] + pi * xr[s];
-            xr[s] = yr;
-            xi[s] = yi;
+            for (int s = 0; s < 16; ++s) {
+                if ((s & cin) != cin) continue;
+                const T yr = pr * xr[s] - pi * xi[s];
+                const T yi = pr * xi[s] + pi * xr[s];
+                xr[s] = yr;
+                xi[s] = yi;
+            }
+        } else if (s1p < 0) {
+            // first selector inside: two candidate phases per thread
+            const T p0r = tb1 ? r2 : r0, p0i = tb1 ? i2 : i0;
+            const T p1r = tb1 ? r3 : r1, p1i = tb1 ? i3 : i1;
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                if ((s & cin) != cin) continue;
+                const bool b = (s >> s0p) & 1;
+                const T pr = b ? p1r : p0r, pi = b ? p1i : p0i;
+                const T yr = pr * xr[s] - pi * xi[s];
+                const T yi = pr * xi[s] + pi * xr[s];
+                xr[s] = yr;
+                xi[s] = yi;
+            }
+        } else {
+            // general: second selector inside (first inside or out)
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                if ((s & cin) != cin) continue;
+                const int b0 = s0p >= 0 ? ((s >> s0p) & 1) : tb0;
+                const int b1 = (s >> s1p) & 1;
+                const T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
+                const T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
+                const T yr = pr * xr[s] - pi * xi[s];
+                const T yi = pr * xi[s] + pi * xr[s];
+                xr[s] = yr;
+                xi[s] = yi;
+            }
         }
     } else if (kind == kTPairReal) {
         g_pair_at<kTPairReal, T>(xr, xi, op, cin);
@@ -1051,13 +1082,5 @@ extern "C" int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, in
 }
 
 namespace lq {
-
-int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
-                        double shift, const PauliSum& ps, double* energy_out, double* grad_out) {
-    (void)n; (void)dtype; (void)tmpl; (void)num_gates; (void)params; (void)num_params; (void)shift; (void)ps;
-    (void)energy_out; (void)grad_out;
-    set_error("batched gradient is not built yet");
-    return LQ_ERR_UNSUPPORTED;
-}
 
 }  // namespace lq
