@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(kThreads) k_dense(const double2* __restrict__ 
 // ---- host launchers -----------------------------------------------------------------
 // Variant knobs (autotuned): work units per wave (2, 4, 8) and nontemporal
 // global access.
-static int g_unroll = 4;
-static bool g_nontemporal = false;
+static int g_unroll = 2;
+static bool g_nontemporal = true;
 
 struct Split {
     ChunkBits<double> fb{};   // chunk-space fixed positions (positions only)

@@ -64,7 +64,7 @@ def test_gate_kernel_variants(hip, unroll, nt):
             assert np.max(np.abs(gpu_apply(psi, n, gates) - want)) <= AMP_TOL
             assert np.max(np.abs(gpu_apply(psi, n, gates, lq.C64) - want)) <= C64_TOL
     finally:
-        lq._lib.call("lq_set_gate_kernel_config", 4, 0)
+        lq._lib.call("lq_set_gate_kernel_config", 2, 1)
 
 
 @pytest.mark.parametrize("n", [7, 12])
