@@ -222,9 +222,21 @@ int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double*
  * LQ_GRAD_BATCHED runs all 2P+1 circuits together in tile passes. */
 #define LQ_GRAD_SEQUENTIAL 0
 #define LQ_GRAD_BATCHED 1
+/* grad_mode of lq_vqe_run only: the adjoint gradient below. */
+#define LQ_GRAD_ADJOINT 2
 int lq_param_shift_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, const double* params,
                             int num_params, double shift, const uint8_t* paulis, const double* coeffs,
                             int num_terms, int convention, int mode, double* energy_out, double* grad_out);
+
+/* Adjoint-method gradient (no counterpart in the reference, which offers
+ * ParameterShift and FiniteDifference): one forward sweep, lambda = H psi,
+ * one backward sweep; dE/dtheta = Im <lambda|P|psi> at each rotation
+ * exp(-i theta/2 P).  Parameters may sit on RX / RY / RZ only; every gate
+ * must have a unitary inverse (no renormalising MatrixGate).  Equals
+ * ParameterShift (shift pi/2) up to rounding. */
+int lq_adjoint_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, const double* params,
+                        int num_params, const uint8_t* paulis, const double* coeffs, int num_terms, int convention,
+                        double* energy_out, double* grad_out);
 
 /* ---- Batches (no reference counterpart: the reference evaluates circuits
  * one after another) ------------------------------------------------------ */

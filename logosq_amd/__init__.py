@@ -12,7 +12,7 @@ from .gates import *  # noqa: F401,F403
 from .circuits import Circuit, Operation, Param, set_default_exec_mode, default_exec_mode
 from .optimization import (Pauli, PauliTerm, PauliObservable, Observable, Ansatz, HardwareEfficientAnsatz,
                            RealAmplitudesAnsatz, EfficientSU2Ansatz, ParameterizedCircuit, EntanglingGate,
-                           EntanglingPattern, ParameterShift, FiniteDifference, GradientDescent, Adam, VQE,
+                           EntanglingPattern, ParameterShift, AdjointGradient, FiniteDifference, GradientDescent, Adam, VQE,
                            heisenberg_ring, ry_rz_cz_ring_ansatz)
 from . import algorithms
 from .batch import batch_expectations

@@ -31,7 +31,7 @@ PAULI_I, PAULI_X, PAULI_Y, PAULI_Z = 0, 1, 2, 3
 PAULI_REFERENCE, PAULI_PHYSICAL = 0, 1
 
 EXEC_PER_GATE, EXEC_FUSED = 0, 1
-GRAD_SEQUENTIAL, GRAD_BATCHED = 0, 1
+GRAD_SEQUENTIAL, GRAD_BATCHED, GRAD_ADJOINT = 0, 1, 2
 
 ARITY = {X: 1, Y: 1, Z: 1, H: 1, S: 1, T: 1, RX: 1, RY: 1, RZ: 1, PHASE: 1, U3: 1, MATRIX1: 1,
          CNOT: 2, SWAP: 2, CZ: 2, CPHASE: 2, CRX: 2, CRY: 2, CRZ: 2, CP: 2, RXX: 2, RYY: 2, RZZ: 2,

@@ -137,8 +137,12 @@ static void vqe_g(const double* x, int dim, double* grad, void* user) {
         for (int i = 0; i < dim; ++i) grad[i] = NAN;
         return;
     }
-    c->rc = lq_param_shift_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->shift, c->paulis, c->coeffs,
-                                    c->num_terms, c->convention, c->grad_mode, nullptr, grad);
+    if (c->grad_mode == LQ_GRAD_ADJOINT)
+        c->rc = lq_adjoint_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->paulis, c->coeffs,
+                                    c->num_terms, c->convention, nullptr, grad);
+    else
+        c->rc = lq_param_shift_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->shift, c->paulis,
+                                        c->coeffs, c->num_terms, c->convention, c->grad_mode, nullptr, grad);
     if (c->rc)
         for (int i = 0; i < dim; ++i) grad[i] = NAN;
 }

@@ -396,6 +396,34 @@ def _energy(ansatz, observable, params):
     return observable.expectation(st)
 
 
+class AdjointGradient:
+    """Adjoint-method gradient on the GPU (no reference counterpart): equal to
+    ParameterShift (shift pi/2) up to rounding for RX/RY/RZ parameters, at
+    the cost of one forward and one backward sweep."""
+
+    shift = math.pi / 2.0
+    batched = False
+    mode = abi.GRAD_ADJOINT
+
+    def __init__(self, dtype=C128):
+        self.dtype = dtype
+
+    def compute_gradient(self, ansatz, observable, parameters, energy=False):
+        arr, ng = _template_args(ansatz)
+        codes, coeffs = _obs_args(observable)
+        p = np.ascontiguousarray(parameters, dtype=np.float64)
+        grad = np.zeros(len(p))
+        e = ctypes.c_double()
+        call("lq_adjoint_gradient", ansatz.num_qubits(), self.dtype, arr, ng,
+             p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(p),
+             codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+             coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(coeffs), observable.convention,
+             ctypes.byref(e) if energy else None, grad.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if energy:
+            return e.value, grad.tolist()
+        return grad.tolist()
+
+
 class FiniteDifference:
     """FiniteDifference (gradient.rs:81-125): forward differences."""
 
@@ -506,7 +534,7 @@ class VQE:
         hist = np.zeros(max(1, opt.max_iterations))
         res = LqOptResult()
         gm = self.gradient_method
-        mode = abi.GRAD_BATCHED if getattr(gm, "batched", False) else abi.GRAD_SEQUENTIAL
+        mode = getattr(gm, "mode", abi.GRAD_BATCHED if getattr(gm, "batched", False) else abi.GRAD_SEQUENTIAL)
         dtype = getattr(gm, "dtype", C128)
         final = State.zero_state(self.ansatz.num_qubits(), dtype)
         call("lq_vqe_run", self.ansatz.num_qubits(), dtype, arr, ng, len(x),

@@ -63,3 +63,39 @@ def test_uneven_batch_matches_oracle(hip, dtype, tol):
         psi = oracle.execute(oracle.zero_state(n), n, gates)
         want = oracle.pauli_sum(psi, n, codes, coeffs)
         assert abs(e - want) <= tol, (n, e, want)
+
+
+@pytest.mark.parametrize("n,layers", [(3, 2), (8, 3), (14, 2)])
+def test_adjoint_gradient_matches_param_shift(hip, n, layers):
+    """The adjoint gradient equals the oracle's parameter-shift gradient
+    (<= 1e-10), on the ring ansatz and on EfficientSU2 with a random Pauli sum
+    holding odd-Y terms (reference convention)."""
+    say("adjoint n=%d" % n)
+    rng = np.random.default_rng(700 + n)
+    for ans, obs in ((lq.ry_rz_cz_ring_ansatz(n, layers), lq.heisenberg_ring(n, *rng.uniform(-1, 1, 3))),
+                     (lq.EfficientSU2Ansatz(n, layers), None)):
+        if obs is None:
+            obs = lq.PauliObservable(n)
+            for _ in range(10):
+                obs.add_term(lq.PauliTerm(rng.normal(), rng.integers(0, 4, n)))
+        x = rng.uniform(0, 2 * math.pi, ans.num_parameters())
+        e, g = lq.AdjointGradient().compute_gradient(ans, obs, x, energy=True)
+        codes, coeffs = obs.codes()
+        t = ans.template()
+        assert abs(e - oracle.energy(n, t, x, codes, coeffs)) <= 1e-10
+        assert np.max(np.abs(np.array(g) - oracle.param_shift(n, t, x, codes, coeffs))) <= 1e-10
+
+
+def test_vqe_adjoint_equals_parameter_shift_trajectory(hip):
+    """VQE::run with the adjoint gradient follows the parameter-shift
+    trajectory (oracle) to rounding."""
+    say("vqe adjoint")
+    n = 5
+    ans = lq.ry_rz_cz_ring_ansatz(n, 2)
+    obs = lq.heisenberg_ring(n, 0.3, 0.9, -0.4)
+    x0 = np.linspace(0.1, 1.9, ans.num_parameters())
+    res = lq.VQE(ans, obs, lq.AdjointGradient(), lq.Adam(0.05, 12)).run(x0)
+    codes, coeffs = obs.codes()
+    xo, eo, ho, conv, final = oracle.vqe_run(n, ans.template(), ans.num_parameters(), codes, coeffs, x0, 1, 0.05, 12)
+    assert np.max(np.abs(np.array(res.convergence_history) - ho)) <= 1e-9
+    assert abs(res.ground_state_energy - eo) <= 1e-9
