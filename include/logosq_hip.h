@@ -277,6 +277,34 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
                int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
                double* params, double* history, lq_state* final_state, lq_opt_result* result);
 
+/* ---- Warm-start network (no reference counterpart: the reference starts
+ * every VQE from caller-given parameters, src/optimization/vqe.rs:55-60) --
+ * A perceptron in -> hidden -> hidden -> out with GELU (erf form) after the
+ * two hidden layers.  f32 master weights; every matrix product rounds its
+ * operands to bf16 and accumulates in f32 (MFMA).  Parameters are one f32
+ * vector: W1[hidden][in], b1, W2[hidden][hidden], b2, W3[out][hidden], b3.
+ * Training minimises the mean squared error with Adam (beta1 0.9, beta2
+ * 0.999, eps 1e-8) on the device. */
+typedef struct lq_mlp lq_mlp;
+/* in_dim <= 8, 16 <= hidden <= 4096, out_dim <= 8192; weights and biases
+ * uniform in +-1/sqrt(fan_in) from SplitMix64(seed). */
+int lq_mlp_create(int in_dim, int hidden, int out_dim, uint64_t seed, lq_mlp** out);
+int lq_mlp_destroy(lq_mlp* net);
+int lq_mlp_num_params(const lq_mlp* net, int64_t* count);
+int lq_mlp_get_params(const lq_mlp* net, float* out, int64_t count);
+/* Sets the weights and resets the Adam moments. */
+int lq_mlp_set_params(lq_mlp* net, const float* in, int64_t count);
+/* y[rows][out] = net(x[rows][in]) */
+int lq_mlp_forward(lq_mlp* net, const float* x, int rows, float* y);
+/* Mean squared error over rows x out and its gradient (no update);
+ * loss and grad (num_params floats) are nullable. */
+int lq_mlp_gradients(lq_mlp* net, const float* x, const float* target, int rows, float* loss, float* grad);
+/* epochs passes of Adam minibatch steps over count samples (shuffled per
+ * epoch by SplitMix64(seed)); loss_history (nullable) receives the mean
+ * minibatch loss of each epoch. */
+int lq_mlp_train(lq_mlp* net, const float* x, const float* target, int count, int batch, int epochs,
+                 float learning_rate, uint64_t seed, float* loss_history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ c_u64 = ctypes.c_uint64
 c_double = ctypes.c_double
 c_void_p = ctypes.c_void_p
 P_double = ctypes.POINTER(ctypes.c_double)
+P_float = ctypes.POINTER(ctypes.c_float)
 P_u8 = ctypes.POINTER(ctypes.c_uint8)
 P_int = ctypes.POINTER(ctypes.c_int)
 P_gate = ctypes.POINTER(abi.LqGate)
@@ -69,6 +70,14 @@ SIGNATURES = {
                          P_double, c_int, P_double, c_void_p],
     "lq_vqe_run": [c_int, c_int, P_gate, c_int, c_int, P_u8, P_double, c_int, c_int, c_double, c_int, c_int,
                    c_double, c_int, c_double, P_double, P_double, c_void_p, c_void_p],
+    "lq_mlp_create": [c_int, c_int, c_int, c_u64, ctypes.POINTER(c_void_p)],
+    "lq_mlp_destroy": [c_void_p],
+    "lq_mlp_num_params": [c_void_p, ctypes.POINTER(ctypes.c_int64)],
+    "lq_mlp_get_params": [c_void_p, P_float, ctypes.c_int64],
+    "lq_mlp_set_params": [c_void_p, P_float, ctypes.c_int64],
+    "lq_mlp_forward": [c_void_p, P_float, c_int, P_float],
+    "lq_mlp_gradients": [c_void_p, P_float, P_float, c_int, P_float, P_float],
+    "lq_mlp_train": [c_void_p, P_float, P_float, c_int, c_int, c_int, ctypes.c_float, c_u64, P_float],
 }
 
 

@@ -33,8 +33,13 @@ def sources():
 EXTRA = {"lq_fused.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
 
 
-def _compile(src):
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+        deps.append(os.path.join(os.path.dirname(HERE), "include", "logosq_hip.h"))
+        if os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps if os.path.exists(d)):
+            return obj
     cmd = [HIPCC, *HIP_FLAGS, *EXTRA.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -42,11 +47,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, incremental=False):
+    """Compile every source (incremental=True: only sources newer than their
+    object file or than any header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,5 +66,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)
