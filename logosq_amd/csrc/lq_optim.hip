@@ -2,6 +2,7 @@
 // Adam (src/optimization/optimizer.rs:35-224) and VQE::run
 // (src/optimization/vqe.rs:55-99), with the objective and gradient evaluated
 // by the GPU path.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -118,15 +119,36 @@ struct VqeCtx {
     const double* coeffs;
     int num_terms, convention;
     int rc;
+    // adjoint / batched modes: the gradient evaluated together with the
+    // energy in vqe_f, kept for the vqe_g call at the same point
+    std::vector<double> cached_x, cached_g;
+    bool cached = false;
 };
 
 int energy_sequential(lq_state* st, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
                       const PauliSum& ps, std::vector<lq_gate>& scratch_gates, double* out);
 
+// Sequential mode evaluates E with per-gate kernels, as the reference's
+// VQE does (vqe.rs:63-70).  The batched and adjoint gradients produce
+// E(x) together with the gradient, so there one call serves both the
+// objective and the gradient request that Adam / GD make at the same x.
 static double vqe_f(const double* x, int dim, void* user) {
     VqeCtx* c = (VqeCtx*)user;
     double e = 0.0;
     if (c->rc) return NAN;
+    if (c->grad_mode == LQ_GRAD_ADJOINT || c->grad_mode == LQ_GRAD_BATCHED) {
+        c->cached_g.assign(dim > 0 ? dim : 1, 0.0);
+        if (c->grad_mode == LQ_GRAD_ADJOINT)
+            c->rc = lq_adjoint_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->paulis, c->coeffs,
+                                        c->num_terms, c->convention, &e, c->cached_g.data());
+        else
+            c->rc = lq_param_shift_gradient(c->n, c->dtype, c->tmpl, c->num_gates, x, dim, c->shift, c->paulis,
+                                            c->coeffs, c->num_terms, c->convention, LQ_GRAD_BATCHED, &e,
+                                            c->cached_g.data());
+        c->cached_x.assign(x, x + dim);
+        c->cached = c->rc == 0;
+        return c->rc ? NAN : e;
+    }
     c->rc = energy_sequential(c->st, c->tmpl, c->num_gates, x, dim, *c->ps, c->sg, &e);
     return c->rc ? NAN : e;
 }
@@ -135,6 +157,10 @@ static void vqe_g(const double* x, int dim, double* grad, void* user) {
     VqeCtx* c = (VqeCtx*)user;
     if (c->rc) {
         for (int i = 0; i < dim; ++i) grad[i] = NAN;
+        return;
+    }
+    if (c->cached && (int)c->cached_x.size() == dim && std::equal(x, x + dim, c->cached_x.begin())) {
+        for (int i = 0; i < dim; ++i) grad[i] = c->cached_g[i];
         return;
     }
     if (c->grad_mode == LQ_GRAD_ADJOINT)

@@ -99,3 +99,55 @@ def test_vqe_adjoint_equals_parameter_shift_trajectory(hip):
     xo, eo, ho, conv, final = oracle.vqe_run(n, ans.template(), ans.num_parameters(), codes, coeffs, x0, 1, 0.05, 12)
     assert np.max(np.abs(np.array(res.convergence_history) - ho)) <= 1e-9
     assert abs(res.ground_state_energy - eo) <= 1e-9
+
+
+def _mixed_template(n, layers):
+    """RX/RY/RZ parameters between fixed gates of most kinds (H, S, T, the
+    reference's Y, CNOT, SWAP, CPHASE, U3, RXX in the quirk orientation,
+    Toffoli, CRY) and a CZ ring."""
+    t, p = [], 0
+    for layer in range(layers):
+        for q in range(n):
+            t.append(abi.make_gate([abi.RX, abi.RY, abi.RZ][(q + layer) % 3], [q], [0.0], [p]))
+            p += 1
+        t += [abi.make_gate(abi.H, [0]), abi.make_gate(abi.S, [1]), abi.make_gate(abi.T, [n - 1]),
+              abi.make_gate(abi.Y, [n // 2]), abi.make_gate(abi.CNOT, [0, n - 1]),
+              abi.make_gate(abi.SWAP, [1, n - 2]), abi.make_gate(abi.CPHASE, [n - 1, 0], [0.7]),
+              abi.make_gate(abi.U3, [2], [0.3, 0.5, 0.9]), abi.make_gate(abi.RXX, [2, 1], [0.4]),
+              abi.make_gate(abi.TOFFOLI, [0, 1, n - 1]), abi.make_gate(abi.CRY, [n - 1, 1], [0.8])]
+        t += [abi.make_gate(abi.CZ, [q, (q + 1) % n]) for q in range(n)]
+    return t, p
+
+
+def _adjoint(n, dtype, t, x, codes, coeffs):
+    import ctypes
+    arr = abi.gate_array(t)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.float64)
+    g = np.zeros(len(x))
+    e = ctypes.c_double()
+    pd = ctypes.POINTER(ctypes.c_double)
+    lq._lib.call("lq_adjoint_gradient", n, dtype, arr, len(t), x.ctypes.data_as(pd), len(x),
+                 codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), coeffs.ctypes.data_as(pd), len(coeffs),
+                 abi.PAULI_REFERENCE, ctypes.byref(e), g.ctypes.data_as(pd))
+    return e.value, g
+
+
+@pytest.mark.parametrize("n,dtype", [(4, lq.C128), (9, lq.C128), (13, lq.C128), (13, lq.C64)])
+def test_adjoint_mixed_gates_matches_param_shift(hip, n, dtype):
+    """The fused backward sweep (inverse ops of every kind, tile passes with
+    high bits at n = 13) against the oracle's parameter shift: 1e-10 for
+    complex128, 1e-5 for complex64."""
+    say("adjoint mixed n=%d dtype=%d" % (n, dtype))
+    rng = np.random.default_rng(1300 + n)
+    t, P = _mixed_template(n, 2)
+    x = rng.uniform(0, 2 * math.pi, P)
+    codes = rng.integers(0, 4, size=(12, n)).astype(np.uint8)
+    coeffs = rng.normal(size=12)
+    e, g = _adjoint(n, dtype, t, x, codes, coeffs)
+    tol = 1e-10 if dtype == lq.C128 else 1e-5
+    assert abs(e - oracle.energy(n, t, x, codes, coeffs)) <= tol
+    want = oracle.param_shift(n, t, x, codes, coeffs)
+    assert np.max(np.abs(g - want)) <= tol
+    assert np.max(np.abs(want)) > 1e-3  # the gradient is not trivially zero
