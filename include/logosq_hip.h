@@ -237,6 +237,14 @@ int lq_param_shift_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int 
 int lq_adjoint_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, const double* params,
                         int num_params, const uint8_t* paulis, const double* coeffs, int num_terms, int convention,
                         double* energy_out, double* grad_out);
+/* The adjoint gradient of `count` instances of one template at once (no
+ * reference counterpart): instance i has parameters params[i*num_params ..]
+ * and Hamiltonian coefficients coeffs[i*num_terms ..] over the shared Pauli
+ * strings `paulis`.  energies (nullable) gets count values, grads
+ * count x num_params.  One launch per tile pass serves all instances. */
+int lq_adjoint_gradient_batch(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int count,
+                              const double* params, int num_params, const uint8_t* paulis, const double* coeffs,
+                              int num_terms, int convention, double* energies, double* grads);
 
 /* ---- Batches (no reference counterpart: the reference evaluates circuits
  * one after another) ------------------------------------------------------ */
@@ -276,6 +284,19 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
                const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
                int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
                double* params, double* history, lq_state* final_state, lq_opt_result* result);
+/* `count` VQE::run calls with Adam and the adjoint gradient, advanced in
+ * lock step (no reference counterpart): instance i starts from
+ * params[i*num_params ..] and minimises the Hamiltonian with coefficients
+ * coeffs[i*num_terms ..] over the shared Pauli strings.  Each instance
+ * follows Adam::minimize (optimizer.rs:139-223) exactly, with its own
+ * stopping tests; every iteration evaluates all unfinished instances with
+ * one lq_adjoint_gradient_batch call.  Outputs per instance: optimal
+ * parameters (in place), history (nullable, count x max_iterations),
+ * energies (optimal value), iterations (history length), converged. */
+int lq_vqe_adam_batch(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+                      const uint8_t* paulis, const double* coeffs, int num_terms, int convention, int count,
+                      double learning_rate, int max_iterations, double tolerance, double* params, double* history,
+                      double* energies, int32_t* iterations, int32_t* converged);
 
 /* ---- Warm-start network (no reference counterpart: the reference starts
  * every VQE from caller-given parameters, src/optimization/vqe.rs:55-60) --

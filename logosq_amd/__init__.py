@@ -13,7 +13,7 @@ from .circuits import Circuit, Operation, Param, set_default_exec_mode, default_
 from .optimization import (Pauli, PauliTerm, PauliObservable, Observable, Ansatz, HardwareEfficientAnsatz,
                            RealAmplitudesAnsatz, EfficientSU2Ansatz, ParameterizedCircuit, EntanglingGate,
                            EntanglingPattern, ParameterShift, AdjointGradient, FiniteDifference, GradientDescent, Adam, VQE,
-                           heisenberg_ring, ry_rz_cz_ring_ansatz)
+                           heisenberg_ring, ry_rz_cz_ring_ansatz, vqe_adam_batch)
 from . import algorithms
 from .batch import batch_expectations
 

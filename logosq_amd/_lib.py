@@ -63,6 +63,10 @@ SIGNATURES = {
                                 c_int, c_int, P_double, P_double],
     "lq_adjoint_gradient": [c_int, c_int, P_gate, c_int, P_double, c_int, P_u8, P_double, c_int, c_int, P_double,
                             P_double],
+    "lq_adjoint_gradient_batch": [c_int, c_int, P_gate, c_int, c_int, P_double, c_int, P_u8, P_double, c_int, c_int,
+                                  P_double, P_double],
+    "lq_vqe_adam_batch": [c_int, c_int, P_gate, c_int, c_int, P_u8, P_double, c_int, c_int, c_int, c_double, c_int,
+                          c_double, P_double, P_double, P_double, P_int, P_int],
     "lq_batch_expectations": [c_int, P_int, ctypes.POINTER(P_gate), P_int, ctypes.POINTER(P_u8),
                               ctypes.POINTER(P_double), P_int, c_int, c_int, P_double],
     "lq_gd_minimize": [c_double, c_int, c_double, OBJ_FN, GRAD_FN, c_void_p, P_double, c_int, P_double, c_void_p],

@@ -28,43 +28,69 @@ namespace lq {
 void bind_template(const lq_gate* tmpl, int num_gates, const double* params, int num_params,
                    std::vector<lq_gate>& out);
 int check_template(int n, const lq_gate* tmpl, int num_gates, int num_params);
-int execute_fused(lq_state* st, const lq_gate* gates, int count);
-
 constexpr int kAThreads = 256;
 constexpr int kAWaves = kAThreads / 64;
 
+// States [count][dim]: amplitude 0 of every instance 1 (|0...0>), or all 0.
 template <typename V>
-__global__ void k_zero(V* __restrict__ a, uint64_t dim) {
+__global__ void k_init_states(V* __restrict__ a, uint64_t dim, uint64_t total, int basis0) {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
         V z;
-        z.x = 0;
+        z.x = (basis0 && (i & (dim - 1)) == 0) ? 1 : 0;
         z.y = 0;
         a[i] = z;
     }
 }
 
 // lambda_i += w_x(i) psi_{i ^ x}, w_x(i) = sum_t (cr_t, ci_t) (-1)^{popc(i & zy_t)}
-// (the matrix element <i|H|i^x> of the group's terms).
+// (the matrix element <i|H|i^x> of the group's terms), for every instance:
+// element e of the batch is amplitude e & (dim - 1) of instance e >> n, whose
+// terms start at terms + instance * tstride.
 template <typename V>
 __global__ void __launch_bounds__(kAThreads) k_pauli_apply_add(const V* __restrict__ psi, V* __restrict__ lam,
-                                                               uint64_t dim, uint64_t x,
-                                                               const PauliTermDev* __restrict__ terms, int nt) {
+                                                               int n, uint64_t total, uint64_t x,
+                                                               const PauliTermDev* __restrict__ terms,
+                                                               int tstride, int nt) {
+    const uint64_t dmask = (1ull << n) - 1ull;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const uint64_t i = e & dmask;
+        const PauliTermDev* tt = terms + (e >> n) * (uint64_t)tstride;
         double wr = 0.0, wi = 0.0;
         for (int t = 0; t < nt; ++t) {
-            bool s = __popcll(i & terms[t].zy) & 1;
-            wr += s ? -terms[t].cr : terms[t].cr;
-            wi += s ? -terms[t].ci : terms[t].ci;
+            bool s = __popcll(i & tt[t].zy) & 1;
+            wr += s ? -tt[t].cr : tt[t].cr;
+            wi += s ? -tt[t].ci : tt[t].ci;
         }
-        V b = psi[i ^ x];
-        V l = lam[i];
+        V b = psi[e ^ x];
+        V l = lam[e];
         double br = (double)b.x, bi = (double)b.y;
         l.x = (decltype(l.x))((double)l.x + (wr * br - wi * bi));
         l.y = (decltype(l.y))((double)l.y + (wr * bi + wi * br));
-        lam[i] = l;
+        lam[e] = l;
     }
+}
+
+// partial[inst * gridDim.x + b] = block b's share of Re <psi_inst | lambda_inst>
+// (blockIdx.y = instance); summed by k_rows_sum in a fixed order.
+template <typename V>
+__global__ void __launch_bounds__(kAThreads) k_re_inner(const V* __restrict__ psi, const V* __restrict__ lam,
+                                                        uint64_t dim, double* __restrict__ partial) {
+    __shared__ double sh[kAThreads];
+    const V* a = psi + (uint64_t)blockIdx.y * dim;
+    const V* b = lam + (uint64_t)blockIdx.y * dim;
+    double s = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kAThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kAThreads + threadIdx.x; i < dim; i += stride)
+        s += (double)a[i].x * (double)b[i].x + (double)a[i].y * (double)b[i].y;
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = kAThreads / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // Per-op gradient bookkeeping of the backward plan.
@@ -382,29 +408,6 @@ static unsigned char* adj_host(size_t bytes) {
     return g_adj_host;
 }
 
-// Cached psi / lambda states (same n and dtype across VQE iterations).
-static lq_state* g_adj_psi = nullptr;
-static lq_state* g_adj_lam = nullptr;
-
-static int adj_states(int n, int dtype, lq_state** psi, lq_state** lam) {
-    if (!g_adj_psi || g_adj_psi->n != n || g_adj_psi->dtype != dtype) {
-        if (g_adj_psi) lq_state_destroy(g_adj_psi);
-        if (g_adj_lam) lq_state_destroy(g_adj_lam);
-        g_adj_psi = g_adj_lam = nullptr;
-        int rc = lq_state_create(n, dtype, &g_adj_psi);
-        if (rc) return rc;
-        rc = lq_state_create(n, dtype, &g_adj_lam);
-        if (rc) {
-            lq_state_destroy(g_adj_psi);
-            g_adj_psi = nullptr;
-            return rc;
-        }
-    }
-    *psi = g_adj_psi;
-    *lam = g_adj_lam;
-    return LQ_OK;
-}
-
 static unsigned grid_of(uint64_t work) {
     uint64_t g = (work + kAThreads - 1) / kAThreads;
     if (g > 8192) g = 8192;
@@ -443,53 +446,95 @@ static bool inverse_gate(const lq_gate& g, lq_gate& inv) {
     }
 }
 
+// Device states of a batch: psi then lambda, [count][dim] each (grown on
+// demand, reused across calls).
+static void* g_adj_states = nullptr;
+static size_t g_adj_states_cap = 0;
+
+static void* adj_states(size_t bytes) {
+    if (bytes <= g_adj_states_cap) return g_adj_states;
+    if (hipStreamSynchronize(stream()) != hipSuccess) return nullptr;
+    if (g_adj_states) (void)hipFree(g_adj_states);
+    g_adj_states = nullptr;
+    g_adj_states_cap = 0;
+    if (hipMalloc(&g_adj_states, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_adj_states_cap = bytes;
+    return g_adj_states;
+}
+
+// First primitive of every gate (each gate here yields exactly one).
+static std::vector<int> first_prim_of(const std::vector<Prim>& prims, int num_gates) {
+    std::vector<int> f(num_gates, -1);
+    for (size_t k = 0; k < prims.size(); ++k)
+        if (prims[k].kind >= 0 && f[prims[k].src] < 0) f[prims[k].src] = (int)k;
+    return f;
+}
+
+// count instances of one template: parameters params[i * num_params ...],
+// Hamiltonian coefficients coeffs[i * num_terms ...] over shared Pauli
+// strings.  One plan per direction serves every instance (the structure does
+// not depend on the angles); each instance gets its own op table.
 template <typename V, typename T>
-static int adjoint_t(int n, int dtype, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
-                     const PauliSum& ps, double* energy_out, double* grad_out) {
-    std::vector<lq_gate> g;
-    bind_template(tmpl, num_gates, params, num_params, g);
-    // the backward circuit: U_G^dagger first
-    std::vector<lq_gate> rev(num_gates);
+static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates, int count, const double* params,
+                           int num_params, const uint8_t* paulis, const double* coeffs, int num_terms,
+                           int convention, double* energies, double* grads) {
+    // template-level checks and gradient slots
     std::vector<int> slot_of(num_gates, -1);
     int nslots = 0;
     for (int k = 0; k < num_gates; ++k) {
-        if ((g[k].flags & LQ_GATE_FLAG_RENORMALIZE) || !inverse_gate(g[k], rev[num_gates - 1 - k])) {
-            set_error("adjoint gradient: gate %d (kind %d) has no unitary inverse here", k, g[k].kind);
+        lq_gate inv;
+        if ((tmpl[k].flags & LQ_GATE_FLAG_RENORMALIZE) || !inverse_gate(tmpl[k], inv)) {
+            set_error("adjoint gradient: gate %d (kind %d) has no unitary inverse here", k, tmpl[k].kind);
             return LQ_ERR_UNSUPPORTED;
         }
         for (int a = 0; a < 3; ++a)
-            if (tmpl[k].param[a] >= 0 && !(a == 0 && (g[k].kind == LQ_GATE_RX || g[k].kind == LQ_GATE_RY ||
-                                                      g[k].kind == LQ_GATE_RZ))) {
+            if (tmpl[k].param[a] >= 0 && !(a == 0 && (tmpl[k].kind == LQ_GATE_RX || tmpl[k].kind == LQ_GATE_RY ||
+                                                      tmpl[k].kind == LQ_GATE_RZ))) {
                 set_error("adjoint gradient supports parameters on RX / RY / RZ only (gate %d)", k);
                 return LQ_ERR_UNSUPPORTED;
             }
         if (tmpl[k].param[0] >= 0) slot_of[k] = nslots++;
     }
-    std::vector<Prim> prims;
-    int rc = gates_to_prims(n, rev.data(), num_gates, prims);
+    // Pauli sums: one structure (the strings), per-instance coefficients
+    PauliSum ps0;
+    int rc = compile_pauli_sum(n, paulis, coeffs, num_terms, convention, &ps0);
     if (rc) return rc;
-    const int K = adj_tile_bits(n, dtype);
-    const int L = std::min(default_low_bits(n, dtype), K);
-    Plan plan;
-    plan_passes(n, prims, K, L, plan);
-    std::vector<AdjMeta> meta(plan.ops.size());
-    for (size_t o = 0; o < plan.ops.size(); ++o) {
+    const int nt = (int)ps0.terms.size();
+    // plans from instance 0
+    std::vector<lq_gate> g, rev(num_gates);
+    bind_template(tmpl, num_gates, params, num_params, g);
+    for (int k = 0; k < num_gates; ++k) inverse_gate(g[k], rev[num_gates - 1 - k]);
+    std::vector<Prim> pf, pb;
+    rc = gates_to_prims(n, g.data(), num_gates, pf);
+    if (!rc) rc = gates_to_prims(n, rev.data(), num_gates, pb);
+    if (rc) return rc;
+    const int Kf = default_tile_bits(n, dtype), Lf = default_low_bits(n, dtype);
+    const int Kb = adj_tile_bits(n, dtype), Lb = std::min(default_low_bits(n, dtype), Kb);
+    Plan plf, plb;
+    plan_passes(n, pf, Kf, Lf, plf);
+    plan_passes(n, pb, Kb, Lb, plb);
+    const std::vector<int> fpf = first_prim_of(pf, num_gates), fpb = first_prim_of(pb, num_gates);
+    std::vector<AdjMeta> meta(plb.ops.size());
+    for (size_t o = 0; o < plb.ops.size(); ++o) {
         // the planner's grouping step relabels 2x2 ops by matrix class for the
         // grouped forward kernel; the backward kernel applies every 2x2 op in
         // the general form (the matrix is complete in m[0..7])
-        TileOp& op = plan.ops[o];
+        TileOp& op = plb.ops[o];
         if (op.kind == kTPairReal || op.kind == kTPairRX) op.kind = kTPairMat;
         if (op.kind < kTPairMat || op.kind > kTQuad) {
             set_error("adjoint plan: unexpected op kind %d", op.kind);
             return LQ_ERR_DEVICE;
         }
-        const int k = num_gates - 1 - plan.op_src[o];
+        const int k = num_gates - 1 - plb.op_src[o];
         AdjMeta m{-1, 0, 0, 0};
         if (k >= 0 && k < num_gates && slot_of[k] >= 0) {
             m.slot = slot_of[k];
             m.pauli = g[k].kind == LQ_GATE_RX ? LQ_PAULI_X : g[k].kind == LQ_GATE_RY ? LQ_PAULI_Y : LQ_PAULI_Z;
             m.gbit = n - 1 - g[k].qubits[0];
-            m.lb = plan.ops[o].lb0;
+            m.lb = op.lb0;
             if (m.pauli != LQ_PAULI_Z && m.lb < 0) {
                 set_error("adjoint plan: rotation %d is not on a tile bit", k);
                 return LQ_ERR_DEVICE;
@@ -497,91 +542,177 @@ static int adjoint_t(int n, int dtype, const lq_gate* tmpl, int num_gates, const
         }
         meta[o] = m;
     }
-    lq_state *psi = nullptr, *lam = nullptr;
-    rc = adj_states(n, dtype, &psi, &lam);
-    if (rc) return rc;
+    for (const Step& st : plf.steps)
+        if (st.normalize) {
+            set_error("adjoint gradient: renormalising step in the forward plan");
+            return LQ_ERR_UNSUPPORTED;
+        }
     const uint64_t dim = 1ull << n;
-    const uint64_t ntiles = dim >> K;
-    const uint64_t nwp = ntiles * kAWaves;
-    // device layout: [ops][meta][terms][partials][results]
+    const size_t amp = sizeof(V);
+    const uint64_t nwp = (dim >> Kb) * kAWaves;
+    const int nbE = (int)std::min<uint64_t>(std::max<uint64_t>(dim / (kAThreads * 16), 1), 64);
+    const size_t nof = plf.ops.size(), nob = plb.ops.size();
+    // instances per chunk: device states and partials within half the free
+    // memory (at most 64 GiB), host op tables within 512 MiB
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
+    const size_t budget = std::min<size_t>(freeb / 2, (size_t)64 << 30);
+    const size_t per_dev = 2 * dim * amp + (size_t)std::max(nslots, 1) * nwp * 8 + 4096;
+    const size_t per_host = (nof + nob) * sizeof(TileOp) + (size_t)std::max(nt, 1) * sizeof(PauliTermDev) + 64;
+    int chunk = (int)std::min<size_t>((size_t)count, std::max<size_t>(1, budget / per_dev));
+    chunk = (int)std::min<size_t>((size_t)chunk, std::max<size_t>(1, ((size_t)512 << 20) / per_host));
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t ob = al(plan.ops.size() * sizeof(TileOp) + 1);
-    const size_t mb = al(meta.size() * sizeof(AdjMeta) + 1);
-    const size_t tb = al(std::max<size_t>(ps.terms.size(), 1) * sizeof(PauliTermDev));
-    const size_t pb = al((size_t)std::max(nslots, 1) * nwp * sizeof(double));
-    const size_t rb = al((size_t)std::max(nslots, 1) * sizeof(double));
-    char* base = adj_buffer(ob + mb + tb + pb + rb);
-    unsigned char* hbase = adj_host(ob + mb + tb + rb);
-    if (!base || !hbase) {
-        set_error("adjoint buffer allocation failed");
-        return LQ_ERR_OUT_OF_MEMORY;
-    }
-    TileOp* d_ops = (TileOp*)base;
-    AdjMeta* d_meta = (AdjMeta*)(base + ob);
-    PauliTermDev* d_terms = (PauliTermDev*)(base + ob + mb);
-    double* d_part = (double*)(base + ob + mb + tb);
-    double* d_res = (double*)(base + ob + mb + tb + pb);
-    // staging: the host buffer may still feed the previous call's copy
-    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-    if (!plan.ops.empty()) std::memcpy(hbase, plan.ops.data(), plan.ops.size() * sizeof(TileOp));
-    if (!meta.empty()) std::memcpy(hbase + ob, meta.data(), meta.size() * sizeof(AdjMeta));
-    if (!ps.terms.empty()) std::memcpy(hbase + ob + mb, ps.terms.data(), ps.terms.size() * sizeof(PauliTermDev));
-    LQ_HIP_CHECK(hipMemcpyAsync(base, hbase, ob + mb + tb, hipMemcpyHostToDevice, stream()));
-    // forward sweep (fused tile passes), then lambda = H psi
-    rc = execute_fused(psi, g.data(), num_gates);
-    if (rc) return rc;
-    if (energy_out) {
-        rc = expect_pauli_sum(psi, ps, energy_out);
-        if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_zero<V>, dim3(grid_of(dim)), dim3(kAThreads), 0, stream(), (V*)lam->d_amps, dim);
-    for (const PauliGroup& grp : ps.groups)
-        hipLaunchKernelGGL(k_pauli_apply_add<V>, dim3(grid_of(dim)), dim3(kAThreads), 0, stream(),
-                           (const V*)psi->d_amps, (V*)lam->d_amps, dim, grp.x,
-                           (const PauliTermDev*)(d_terms + grp.first), grp.count);
-    LQ_LAUNCH_CHECK();
-    // backward sweep
-    for (const Step& s : plan.steps) {
-        if (s.normalize) continue;  // no renormalising gates reach here
-        rc = dispatch_adj<V, T>(K, psi->d_amps, lam->d_amps, s.pass, d_ops, d_meta, 1, 0, 0, d_part, nslots, nwp);
-        if (rc) return rc;
-    }
-    if (nslots > 0) {
-        hipLaunchKernelGGL(k_rows_sum, dim3(nslots), dim3(kAThreads), 0, stream(), (const double*)d_part, nwp,
-                           d_res);
+    std::vector<lq_gate> gi, ri(num_gates);
+    std::vector<Prim> pfi, pbi;
+    PauliSum psi_sum;
+    for (int c0 = 0; c0 < count; c0 += chunk) {
+        const int cn = std::min(chunk, count - c0);
+        const size_t b_of = al(cn * nof * sizeof(TileOp) + 1), b_ob = al(cn * nob * sizeof(TileOp) + 1);
+        const size_t b_gr = al(plf.groups.size() * sizeof(TileGroup) + 1);
+        const size_t b_me = al(meta.size() * sizeof(AdjMeta) + 1);
+        const size_t b_te = al((size_t)cn * std::max(nt, 1) * sizeof(PauliTermDev));
+        const size_t b_pa = al((size_t)cn * std::max(nslots, 1) * nwp * sizeof(double));
+        const size_t b_pe = al((size_t)cn * nbE * sizeof(double));
+        const size_t b_re = al((size_t)cn * (std::max(nslots, 1) + 1) * sizeof(double));
+        const size_t up = b_of + b_ob + b_gr + b_me + b_te;
+        char* base = adj_buffer(up + b_pa + b_pe + b_re);
+        unsigned char* hb = adj_host(up + b_re);
+        void* sts = adj_states((size_t)2 * cn * dim * amp);
+        if (!base || !hb || !sts) {
+            set_error("adjoint buffer allocation failed");
+            return LQ_ERR_OUT_OF_MEMORY;
+        }
+        // the staging buffer may still feed the previous chunk's copy
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        TileOp* h_of = (TileOp*)hb;
+        TileOp* h_ob = (TileOp*)(hb + b_of);
+        PauliTermDev* h_te = (PauliTermDev*)(hb + b_of + b_ob + b_gr + b_me);
+        for (int i = 0; i < cn; ++i) {
+            const int inst = c0 + i;
+            bind_template(tmpl, num_gates, params + (size_t)inst * num_params, num_params, gi);
+            for (int k = 0; k < num_gates; ++k) inverse_gate(gi[k], ri[num_gates - 1 - k]);
+            pfi.clear();  // gates_to_prims appends
+            pbi.clear();
+            rc = gates_to_prims(n, gi.data(), num_gates, pfi);
+            if (!rc) rc = gates_to_prims(n, ri.data(), num_gates, pbi);
+            if (!rc) rc = compile_pauli_sum(n, paulis, coeffs + (size_t)inst * num_terms, num_terms, convention,
+                                            &psi_sum);
+            if (rc) return rc;
+            if (pfi.size() != pf.size() || pbi.size() != pb.size() || (int)psi_sum.terms.size() != nt) {
+                set_error("adjoint batch: instance %d differs in structure", inst);
+                return LQ_ERR_INVALID_ARGUMENT;
+            }
+            for (size_t o = 0; o < nof; ++o) {
+                TileOp t = plf.ops[o];
+                std::memcpy(t.m, pfi[fpf[plf.op_src[o]]].m, sizeof(t.m));
+                h_of[(size_t)i * nof + o] = t;
+            }
+            for (size_t o = 0; o < nob; ++o) {
+                TileOp t = plb.ops[o];
+                std::memcpy(t.m, pbi[fpb[plb.op_src[o]]].m, sizeof(t.m));
+                h_ob[(size_t)i * nob + o] = t;
+            }
+            if (nt) std::memcpy(h_te + (size_t)i * nt, psi_sum.terms.data(), nt * sizeof(PauliTermDev));
+        }
+        if (!plf.groups.empty())
+            std::memcpy(hb + b_of + b_ob, plf.groups.data(), plf.groups.size() * sizeof(TileGroup));
+        if (!meta.empty()) std::memcpy(hb + b_of + b_ob + b_gr, meta.data(), meta.size() * sizeof(AdjMeta));
+        LQ_HIP_CHECK(hipMemcpyAsync(base, hb, up, hipMemcpyHostToDevice, stream()));
+        const TileOp* d_of = (const TileOp*)base;
+        const TileOp* d_ob = (const TileOp*)(base + b_of);
+        const TileGroup* d_gr = (const TileGroup*)(base + b_of + b_ob);
+        const AdjMeta* d_me = (const AdjMeta*)(base + b_of + b_ob + b_gr);
+        const PauliTermDev* d_te = (const PauliTermDev*)(base + b_of + b_ob + b_gr + b_me);
+        double* d_pa = (double*)(base + up);
+        double* d_pe = (double*)(base + up + b_pa);
+        double* d_res = (double*)(base + up + b_pa + b_pe);  // [cn * nslots] gradients, then [cn] energies
+        V* psi = (V*)sts;
+        V* lam = psi + (size_t)cn * dim;
+        const uint64_t total = (uint64_t)cn * dim;
+        // forward sweep from |0...0>
+        hipLaunchKernelGGL(k_init_states<V>, dim3(grid_of(total)), dim3(kAThreads), 0, stream(), psi, dim, total, 1);
         LQ_LAUNCH_CHECK();
+        for (const Step& st : plf.steps) {
+            rc = launch_tile_pass(psi, dtype, st.pass, d_of, d_gr, cn, dim, nof);
+            if (rc) return rc;
+        }
+        // lambda = H psi, E = Re <psi|lambda>
+        hipLaunchKernelGGL(k_init_states<V>, dim3(grid_of(total)), dim3(kAThreads), 0, stream(), lam, dim, total, 0);
+        for (const PauliGroup& grp : ps0.groups)
+            hipLaunchKernelGGL(k_pauli_apply_add<V>, dim3(grid_of(total)), dim3(kAThreads), 0, stream(),
+                               (const V*)psi, lam, n, total, grp.x, d_te + grp.first, nt, grp.count);
+        LQ_LAUNCH_CHECK();
+        double* d_e = d_res + (size_t)cn * std::max(nslots, 1);
+        if (energies) {
+            hipLaunchKernelGGL(k_re_inner<V>, dim3(nbE, cn), dim3(kAThreads), 0, stream(), (const V*)psi,
+                               (const V*)lam, dim, d_pe);
+            hipLaunchKernelGGL(k_rows_sum, dim3(cn), dim3(kAThreads), 0, stream(), (const double*)d_pe,
+                               (uint64_t)nbE, d_e);
+            LQ_LAUNCH_CHECK();
+        }
+        // backward sweep
+        for (const Step& st : plb.steps) {
+            rc = dispatch_adj<V, T>(Kb, psi, lam, st.pass, d_ob, d_me, cn, dim, nob, d_pa, nslots, nwp);
+            if (rc) return rc;
+        }
+        if (nslots > 0) {
+            hipLaunchKernelGGL(k_rows_sum, dim3(cn * nslots), dim3(kAThreads), 0, stream(), (const double*)d_pa, nwp,
+                               d_res);
+            LQ_LAUNCH_CHECK();
+        }
+        double* h_res = (double*)(hb + up);
+        const size_t nres = (size_t)cn * (std::max(nslots, 1) + 1);
+        LQ_HIP_CHECK(hipMemcpyAsync(h_res, d_res, nres * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        for (int i = 0; i < cn; ++i) {
+            double* gr = grads + (size_t)(c0 + i) * num_params;
+            for (int p = 0; p < num_params; ++p) gr[p] = 0.0;
+            for (int k = 0; k < num_gates; ++k)
+                if (slot_of[k] >= 0) gr[tmpl[k].param[0]] += h_res[(size_t)i * nslots + slot_of[k]];
+            if (energies) energies[c0 + i] = h_res[(size_t)cn * std::max(nslots, 1) + i];
+        }
     }
-    double* hres = (double*)(hbase + ob + mb + tb);
-    if (nslots > 0)
-        LQ_HIP_CHECK(hipMemcpyAsync(hres, d_res, nslots * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-    for (int i = 0; i < num_params; ++i) grad_out[i] = 0.0;
-    for (int k = 0; k < num_gates; ++k)
-        if (slot_of[k] >= 0) grad_out[tmpl[k].param[0]] += hres[slot_of[k]];
     return LQ_OK;
 }
-
 }  // namespace lq
 
 using namespace lq;
 
-extern "C" int lq_adjoint_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates,
-                                   const double* params, int num_params, const uint8_t* paulis, const double* coeffs,
-                                   int num_terms, int convention, double* energy_out, double* grad_out) {
+static int adjoint_entry(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int count,
+                         const double* params, int num_params, const uint8_t* paulis, const double* coeffs,
+                         int num_terms, int convention, double* energies, double* grads) {
     clear_error();
-    if (num_qubits < 1 || num_qubits > kMaxQubits || num_params < 0 || (num_params > 0 && (!params || !grad_out)) ||
-        (dtype != LQ_C128 && dtype != LQ_C64)) {
+    if (num_qubits < 1 || num_qubits > kMaxQubits || num_params < 0 || count < 0 ||
+        (count > 0 && num_params > 0 && (!params || !grads)) || (dtype != LQ_C128 && dtype != LQ_C64) ||
+        num_terms < 0 || (num_terms > 0 && (!paulis || !coeffs))) {
         set_error("invalid arguments");
         return LQ_ERR_INVALID_ARGUMENT;
     }
+    if (count == 0) return LQ_OK;
     int rc = check_template(num_qubits, tmpl, num_gates, num_params);
     if (rc) return rc;
-    PauliSum ps;
-    rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
+    rc = ensure_init();
     if (rc) return rc;
     if (dtype == LQ_C128)
-        return adjoint_t<double2, double>(num_qubits, dtype, tmpl, num_gates, params, num_params, ps, energy_out,
-                                          grad_out);
-    return adjoint_t<float2, float>(num_qubits, dtype, tmpl, num_gates, params, num_params, ps, energy_out,
-                                    grad_out);
+        return adjoint_batch_t<double2, double>(num_qubits, dtype, tmpl, num_gates, count, params, num_params, paulis,
+                                                coeffs, num_terms, convention, energies, grads);
+    return adjoint_batch_t<float2, float>(num_qubits, dtype, tmpl, num_gates, count, params, num_params, paulis,
+                                          coeffs, num_terms, convention, energies, grads);
+}
+
+extern "C" int lq_adjoint_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates,
+                                   const double* params, int num_params, const uint8_t* paulis, const double* coeffs,
+                                   int num_terms, int convention, double* energy_out, double* grad_out) {
+    std::vector<double> dummy;
+    if (num_params == 0) dummy.resize(1);
+    return adjoint_entry(num_qubits, dtype, tmpl, num_gates, 1, params, num_params, paulis, coeffs, num_terms,
+                         convention, energy_out, num_params ? grad_out : dummy.data());
+}
+
+extern "C" int lq_adjoint_gradient_batch(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int count,
+                                         const double* params, int num_params, const uint8_t* paulis,
+                                         const double* coeffs, int num_terms, int convention, double* energies,
+                                         double* grads) {
+    return adjoint_entry(num_qubits, dtype, tmpl, num_gates, count, params, num_params, paulis, coeffs, num_terms,
+                         convention, energies, grads);
 }

@@ -256,4 +256,103 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
     return rc;
 }
 
+int lq_vqe_adam_batch(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+                      const uint8_t* paulis, const double* coeffs, int num_terms, int convention, int count,
+                      double learning_rate, int max_iterations, double tolerance, double* params, double* history,
+                      double* energies, int32_t* iterations, int32_t* converged) {
+    clear_error();
+    if (count < 0 || num_params < 0 || max_iterations < 0 || (count > 0 && (!params || !energies || !iterations ||
+                                                                            !converged || !coeffs))) {
+        set_error("invalid VQE batch arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    int rc = check_template(num_qubits, tmpl, num_gates, num_params);
+    if (rc) return rc;
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+    const size_t P = (size_t)num_params;
+    // per instance: Adam moments, history length, previous value, iteration,
+    // and phase 0 = iterating, 1 = final evaluation pending, 2 = done
+    std::vector<double> m((size_t)count * P, 0.0), v((size_t)count * P, 0.0), prev(count, 0.0);
+    std::vector<int> hist(count, 0), it(count, 1), phase(count, max_iterations > 0 ? 0 : 1);
+    std::vector<char> have_prev(count, 0), conv(count, 0);
+    std::vector<int> act;
+    std::vector<double> xb, cb, eb, gb;
+    for (;;) {
+        act.clear();
+        for (int i = 0; i < count; ++i)
+            if (phase[i] < 2) act.push_back(i);
+        if (act.empty()) break;
+        const int na = (int)act.size();
+        xb.resize((size_t)na * P + 1);
+        cb.resize((size_t)na * num_terms + 1);
+        eb.resize(na);
+        gb.resize((size_t)na * P + 1);
+        for (int a = 0; a < na; ++a) {
+            std::copy(params + (size_t)act[a] * P, params + (size_t)act[a] * P + P, xb.begin() + (size_t)a * P);
+            std::copy(coeffs + (size_t)act[a] * num_terms, coeffs + (size_t)act[a] * num_terms + num_terms,
+                      cb.begin() + (size_t)a * num_terms);
+        }
+        rc = lq_adjoint_gradient_batch(num_qubits, dtype, tmpl, num_gates, na, xb.data(), num_params, paulis,
+                                       cb.data(), num_terms, convention, eb.data(), gb.data());
+        if (rc) return rc;
+        for (int a = 0; a < na; ++a) {
+            const int i = act[a];
+            double* x = params + (size_t)i * P;
+            const double* g = gb.data() + (size_t)a * P;
+            const double value = eb[a];
+            double gn = 0.0;
+            for (size_t k = 0; k < P; ++k) gn += g[k] * g[k];
+            gn = std::sqrt(gn);
+            if (phase[i] == 1) {  // the evaluation after the loop (optimizer.rs:208-214)
+                energies[i] = value;
+                if (gn < tolerance) conv[i] = 1;
+                phase[i] = 2;
+                continue;
+            }
+            if (history) history[(size_t)i * max_iterations + hist[i]] = value;
+            ++hist[i];
+            // the loop left at this x: the final evaluation is this one
+            bool stop = (have_prev[i] && std::fabs(prev[i] - value) < tolerance);
+            if (!stop) {
+                prev[i] = value;
+                have_prev[i] = 1;
+                stop = gn < tolerance;
+            }
+            if (stop) {
+                conv[i] = 1;
+                energies[i] = value;
+                phase[i] = 2;
+                continue;
+            }
+            double* mi = m.data() + (size_t)i * P;
+            double* vi = v.data() + (size_t)i * P;
+            for (size_t k = 0; k < P; ++k) {
+                mi[k] = b1 * mi[k] + (1.0 - b1) * g[k];
+                vi[k] = b2 * vi[k] + (1.0 - b2) * g[k] * g[k];
+            }
+            const double mc = 1.0 / (1.0 - powi_f64(b1, it[i]));
+            const double vc = 1.0 / (1.0 - powi_f64(b2, it[i]));
+            double sn = 0.0;
+            for (size_t k = 0; k < P; ++k) {
+                const double step = learning_rate * (mi[k] * mc) / (std::sqrt(vi[k] * vc) + eps);
+                x[k] -= step;
+                sn += step * step;
+            }
+            if (std::sqrt(sn) < tolerance) {
+                conv[i] = 1;
+                phase[i] = 1;
+            } else if (it[i] == max_iterations) {
+                phase[i] = 1;
+            } else {
+                ++it[i];
+            }
+        }
+    }
+    for (int i = 0; i < count; ++i) {
+        iterations[i] = hist[i];
+        converged[i] = conv[i];
+    }
+    return LQ_OK;
+}
+
 }  // extern "C"

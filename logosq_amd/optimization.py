@@ -424,6 +424,60 @@ class AdjointGradient:
         return grad.tolist()
 
 
+    def compute_gradient_batch(self, ansatz, observables, parameter_sets, energy=False):
+        """Adjoint gradients of many (observable, parameters) pairs of one
+        ansatz in one batched call; the observables must share their Pauli
+        strings (only the coefficients differ)."""
+        arr, ng = _template_args(ansatz)
+        codes, coeff_sets = _shared_strings(observables)
+        x = np.ascontiguousarray(parameter_sets, dtype=np.float64).reshape(len(observables), -1)
+        grads = np.zeros_like(x)
+        e = np.zeros(len(observables))
+        pd = ctypes.POINTER(ctypes.c_double)
+        call("lq_adjoint_gradient_batch", ansatz.num_qubits(), self.dtype, arr, ng, len(observables),
+             x.ctypes.data_as(pd), x.shape[1], codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+             coeff_sets.ctypes.data_as(pd), coeff_sets.shape[1], observables[0].convention,
+             e.ctypes.data_as(pd), grads.ctypes.data_as(pd))
+        return (e, grads) if energy else grads
+
+
+def _shared_strings(observables):
+    """(codes, coefficient matrix) of observables with identical Pauli strings."""
+    codes0, _ = _obs_args(observables[0])
+    coeff_sets = []
+    for o in observables:
+        c, k = _obs_args(o)
+        if c.shape != codes0.shape or not np.array_equal(c, codes0) or o.convention != observables[0].convention:
+            raise ValueError("batched observables must share their Pauli strings and convention")
+        coeff_sets.append(k)
+    return codes0, np.ascontiguousarray(np.array(coeff_sets, dtype=np.float64).reshape(len(observables), -1))
+
+
+def vqe_adam_batch(ansatz, hamiltonians, initial_parameters, optimizer, dtype=C128):
+    """VQE::run with Adam and the adjoint gradient for many Hamiltonians of
+    one family (shared Pauli strings) in lock step on the GPU; every run
+    follows Adam::minimize exactly.  Returns a list of VQEResult (without
+    final states) plus the converged flags."""
+    arr, ng = _template_args(ansatz)
+    codes, coeff_sets = _shared_strings(hamiltonians)
+    count = len(hamiltonians)
+    x = np.array(initial_parameters, dtype=np.float64, copy=True).reshape(count, -1)
+    it_max = int(optimizer.max_iterations)
+    hist = np.zeros((count, max(1, it_max)))
+    e = np.zeros(count)
+    iters = np.zeros(count, dtype=np.int32)
+    conv = np.zeros(count, dtype=np.int32)
+    pd = ctypes.POINTER(ctypes.c_double)
+    pi = ctypes.POINTER(ctypes.c_int)
+    call("lq_vqe_adam_batch", ansatz.num_qubits(), dtype, arr, ng, x.shape[1],
+         codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), coeff_sets.ctypes.data_as(pd), coeff_sets.shape[1],
+         hamiltonians[0].convention, count, optimizer.learning_rate, it_max, optimizer.tolerance,
+         x.ctypes.data_as(pd), hist.ctypes.data_as(pd), e.ctypes.data_as(pd), iters.ctypes.data_as(pi),
+         conv.ctypes.data_as(pi))
+    out = [VQEResult(e[i], x[i].tolist(), hist[i, :iters[i]].tolist(), None) for i in range(count)]
+    return out, conv.astype(bool)
+
+
 class FiniteDifference:
     """FiniteDifference (gradient.rs:81-125): forward differences."""
 

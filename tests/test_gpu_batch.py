@@ -151,3 +151,58 @@ def test_adjoint_mixed_gates_matches_param_shift(hip, n, dtype):
     want = oracle.param_shift(n, t, x, codes, coeffs)
     assert np.max(np.abs(g - want)) <= tol
     assert np.max(np.abs(want)) > 1e-3  # the gradient is not trivially zero
+
+
+@pytest.mark.parametrize("n,layers", [(5, 2), (12, 2)])
+def test_adjoint_batch_equals_single_and_oracle(hip, n, layers):
+    """Batched adjoint gradients (many couplings and parameter sets of one
+    ansatz) equal the single-instance ones and the oracle's parameter shift."""
+    say("adjoint batch n=%d" % n)
+    rng = np.random.default_rng(4100 + n)
+    ans = lq.ry_rz_cz_ring_ansatz(n, layers)
+    P = ans.num_parameters()
+    Js = rng.uniform(-1, 1, (5, 3))
+    obs = [lq.heisenberg_ring(n, *J) for J in Js]
+    X = rng.uniform(0, 2 * math.pi, (5, P))
+    adj = lq.AdjointGradient()
+    E, G = adj.compute_gradient_batch(ans, obs, X, energy=True)
+    for i in range(5):
+        e1, g1 = adj.compute_gradient(ans, obs[i], X[i], energy=True)
+        assert abs(E[i] - e1) <= 1e-14 and np.max(np.abs(G[i] - np.array(g1))) <= 1e-14
+    t = ans.template()
+    for i in (0, 4):
+        codes, coeffs = obs[i].codes()
+        assert abs(E[i] - oracle.energy(n, t, X[i], codes, coeffs)) <= 1e-10
+        assert np.max(np.abs(G[i] - oracle.param_shift(n, t, X[i], codes, coeffs))) <= 1e-10
+    E64, G64 = lq.AdjointGradient(lq.C64).compute_gradient_batch(ans, obs, X, energy=True)
+    assert np.max(np.abs(E64 - E)) <= 1e-5 and np.max(np.abs(G64 - G)) <= 1e-5
+
+
+def test_vqe_adam_batch_equals_single_runs(hip):
+    """Lock-step VQE over several couplings: each instance's history,
+    optimum and stopping match its own VQE::run (adjoint gradient), and one
+    matches the oracle's parameter-shift VQE."""
+    say("vqe adam batch")
+    n = 6
+    ans = lq.ry_rz_cz_ring_ansatz(n, 2)
+    rng = np.random.default_rng(4200)
+    Js = rng.uniform(-1, 1, (4, 3))
+    obs = [lq.heisenberg_ring(n, *J) for J in Js]
+    X0 = rng.uniform(0, 2 * math.pi, (4, ans.num_parameters()))
+    for lr, it, tol in ((0.05, 40, 1e-6), (0.1, 60, 2e-3)):
+        res, conv = lq.vqe_adam_batch(ans, obs, X0, lq.Adam(lr, it).with_tolerance(tol))
+        iters = []
+        for i in range(4):
+            single = lq.VQE(ans, obs[i], lq.AdjointGradient(), lq.Adam(lr, it).with_tolerance(tol)).run(X0[i])
+            assert len(single.convergence_history) == len(res[i].convergence_history)
+            assert np.max(np.abs(np.array(single.convergence_history) - res[i].convergence_history)) <= 1e-12
+            assert np.max(np.abs(np.array(single.optimal_parameters) - res[i].optimal_parameters)) <= 1e-12
+            assert abs(single.ground_state_energy - res[i].ground_state_energy) <= 1e-12
+            iters.append(len(res[i].convergence_history))
+        if tol > 1e-3:
+            assert min(iters) < it and conv.any()  # instances stop on their own
+    codes, coeffs = obs[0].codes()
+    res, _ = lq.vqe_adam_batch(ans, obs, X0, lq.Adam(0.05, 25))
+    xo, eo, ho, co, _ = oracle.vqe_run(n, ans.template(), ans.num_parameters(), codes, coeffs, X0[0], 1, 0.05, 25)
+    assert np.max(np.abs(np.array(res[0].convergence_history) - ho)) <= 1e-9
+    assert abs(res[0].ground_state_energy - eo) <= 1e-9
