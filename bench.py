@@ -26,6 +26,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def traffic_from_profiles(mode):
+    """HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    --pmc (FETCH_SIZE and WRITE_SIZE runs, scaled by the same counters on a
+    copy kernel of known size) and recorded in profiles/; None if absent."""
+    name = {"per_gate": "traffic_k_pair_rx_30q_c128.txt", "fused": "traffic_k_tileg_30q_c128.txt"}[mode]
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -35,9 +50,9 @@ def parse():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
-    ap.add_argument("--mode", choices=["per_gate", "fused"], default="fused")
+    ap.add_argument("--mode", choices=["per_gate", "fused"], default="per_gate")
     ap.add_argument("--cpu-qubits", type=int, default=30)
-    ap.add_argument("--cpu-gates", type=int, default=3)
+    ap.add_argument("--cpu-gates", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the gradient / uneven-batch workloads")
@@ -163,7 +178,8 @@ def cpu_baseline(nq, ngates, seed, depth):
     lib = oracle.lib()
     descs = algorithms.brickwork_circuit(nq, depth, seed).descriptors()
     layer0 = descs[:nq] + descs[nq:nq + nq // 2]
-    sample = [layer0[0], layer0[nq // 2], layer0[nq]][:ngates] if ngates <= 3 else layer0[:ngates]
+    nrot = (2 * ngates) // 3
+    sample = layer0[:min(nrot, nq)] + layer0[nq:nq + ngates - min(nrot, nq)]
     v = np.zeros(1 << nq, dtype=np.complex128)
     v[:] = 0  # touch every page before timing
     v[0] = 1
@@ -178,6 +194,36 @@ def cpu_baseline(nq, ngates, seed, depth):
             "sample": "%d gates of layer 0 (%s) at %d qubits c128, oracle gate loops single-threaded as the "
                       "reference's, %.1f s" % (len(sample), ",".join(names.get(g.kind, str(g.kind)) for g in sample),
                                               nq, dt)}
+
+
+def run_circuit(lq, st, arr, count, mode, steps, warmup, label):
+    from logosq_amd import abi
+    for _ in range(warmup):
+        lq._lib.call("lq_state_set_basis", st.handle, 0)
+        lq._lib.call("lq_execute", st.handle, arr, count, mode)
+    lq._lib.call("lq_synchronize")
+    times = []
+    for _ in range(steps):
+        lq._lib.call("lq_state_set_basis", st.handle, 0)
+        lq._lib.call("lq_synchronize")
+        t0 = time.perf_counter()
+        lq._lib.call("lq_execute", st.handle, arr, count, mode)
+        lq._lib.call("lq_synchronize")
+        times.append(time.perf_counter() - t0)
+        print("bench: %s step %.3f s" % (label, times[-1]), file=sys.stderr, flush=True)
+    return times
+
+
+def event_time(lq, fn):
+    """GPU time (s) of the launches fn() queues, from HIP events recorded on
+    the library stream around them."""
+    lq._lib.call("lq_synchronize")
+    lq._lib.call("lq_event_record", 2)
+    fn()
+    lq._lib.call("lq_event_record", 3)
+    ms = ctypes.c_float()
+    lq._lib.call("lq_event_elapsed_ms", 2, 3, ctypes.byref(ms))
+    return ms.value / 1e3
 
 
 def main():
@@ -202,59 +248,61 @@ def main():
     descs = circ.descriptors()
     arr = abi.gate_array(descs)
     st = lq.State(a.qubits, dtype)
+    N = 1 << a.qubits
     print("bench: %d qubits %s depth %d, %d gates, mode %s" % (a.qubits, a.dtype, a.depth, len(descs), a.mode),
           file=sys.stderr, flush=True)
-    for _ in range(a.warmup):
-        lq._lib.call("lq_state_set_basis", st.handle, 0)
-        lq._lib.call("lq_execute", st.handle, arr, len(descs), mode)
-    lq._lib.call("lq_synchronize")
-    times = []
-    for _ in range(a.steps):
-        lq._lib.call("lq_state_set_basis", st.handle, 0)
-        lq._lib.call("lq_synchronize")
-        t0 = time.perf_counter()
-        lq._lib.call("lq_execute", st.handle, arr, len(descs), mode)
-        lq._lib.call("lq_synchronize")
-        times.append(time.perf_counter() - t0)
-        print("bench: step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    times = run_circuit(lq, st, arr, len(descs), mode, a.steps, a.warmup, a.mode)
     norm2 = st.norm_sqr()
     step_s = sum(times) / len(times)
     bytes_gates = algorithmic_bytes(descs, a.qubits, c)
-    N = 1 << a.qubits
+    passes = num_passes(lq, a.qubits, dtype, descs)
     if mode == abi.EXEC_FUSED:
-        passes = num_passes(lq, a.qubits, dtype, descs)
-        # one fused execution bracketed by events: every launch is a tile pass
-        lq._lib.call("lq_state_set_basis", st.handle, 0)
-        lq._lib.call("lq_synchronize")
-        lq._lib.call("lq_event_record", 2)
-        lq._lib.call("lq_execute", st.handle, arr, len(descs), mode)
-        lq._lib.call("lq_event_record", 3)
-        ms = ctypes.c_float()
-        lq._lib.call("lq_event_elapsed_ms", 2, 3, ctypes.byref(ms))
-        avg_s = ms.value / 1e3 / passes
         bytes_moved = passes * 2 * c * N
-        kname = "k_tile (fused LDS tile pass), %d passes per circuit" % passes
-        bpl = 2 * c * N
     else:
-        passes = len(descs)
-        avg_s = None
         bytes_moved = bytes_gates
     value = bytes_moved / step_s / 1e9
     # per-gate kernels: RX on every target (pair kernel) and the circuit's CNOTs
     rx = [abi.make_gate(abi.RX, [q], [0.1 + q]) for q in range(a.qubits)]
     cn = [g for g in descs if g.kind == abi.CNOT][: a.qubits // 2]
-    t_rx = time_kernels(lq, st, rx)
-    t_cn = time_kernels(lq, st, cn)
+    rx_arr, cn_arr = abi.gate_array(rx), abi.gate_array(cn)
+    t_rx = event_time(lq, lambda: lq._lib.call("lq_execute", st.handle, rx_arr, len(rx), abi.EXEC_PER_GATE))
+    t_cn = event_time(lq, lambda: lq._lib.call("lq_execute", st.handle, cn_arr, len(cn), abi.EXEC_PER_GATE))
     rx_gbs = 2 * c * N * len(rx) / t_rx / 1e9
     cn_gbs = c * N * len(cn) / t_cn / 1e9
-    if avg_s is None:
-        avg_s = t_rx / len(rx)
-        bpl = 2 * c * N
-        kname = "k_pair (RX on each of the %d targets)" % a.qubits
+    # the other execution mode of the same circuit
+    other = abi.EXEC_FUSED if mode == abi.EXEC_PER_GATE else abi.EXEC_PER_GATE
+    oname = "fused" if other == abi.EXEC_FUSED else "per_gate"
+    t_other = run_circuit(lq, st, arr, len(descs), other, 1, 1 if other == abi.EXEC_FUSED else 0, oname)[0]
+    t_pass = event_time(lq, lambda: (lq._lib.call("lq_state_set_basis", st.handle, 0),
+                                     lq._lib.call("lq_execute", st.handle, arr, len(descs), abi.EXEC_FUSED)))
+    t_fill = event_time(lq, lambda: lq._lib.call("lq_state_set_basis", st.handle, 0))
+    pass_s = max(t_pass - t_fill, 1e-9) / passes
+    fused_s = t_other if other == abi.EXEC_FUSED else step_s
+    per_gate_s = step_s if mode == abi.EXEC_PER_GATE else t_other
+    if mode == abi.EXEC_PER_GATE:
+        kname = "k_pair_hi / k_pair_lo (RX on each of the %d targets, per-gate path)" % a.qubits
+        bpl, avg_s = 2 * c * N, t_rx / len(rx)
+    else:
+        kname = "k_tileg (fused LDS tile pass), %d passes per circuit" % passes
+        bpl, avg_s = 2 * c * N, pass_s
     achieved = bpl / avg_s / 1e9
+    fused = {"ms_per_circuit": round(fused_s * 1e3, 3), "passes": passes,
+             "speedup_vs_per_gate": round(per_gate_s / fused_s, 2),
+             "effective_gate_GBps": round(bytes_gates / fused_s / 1e9, 1),
+             "pass_kernel_GBps": round(2 * c * N / pass_s / 1e9, 1),
+             "pass_kernel_hbm_frac": round(2 * c * N / pass_s / 1e9 / HBM_PEAK_GBS, 4)}
+    st = None
+    c64 = None
+    if dtype == lq.C128 and not a.no_extra:
+        # the same per-gate kernels on a complex64 state
+        s64 = lq.State(a.qubits, lq.C64)
+        t64 = event_time(lq, lambda: lq._lib.call("lq_execute", s64.handle, rx_arr, len(rx), abi.EXEC_PER_GATE))
+        t64c = event_time(lq, lambda: lq._lib.call("lq_execute", s64.handle, cn_arr, len(cn), abi.EXEC_PER_GATE))
+        c64 = {"rx_pair_GBps": round(2 * 8 * N * len(rx) / t64 / 1e9, 1),
+               "cnot_GBps": round(8 * N * len(cn) / t64c / 1e9, 1)}
+        s64 = None
     extra = {}
     if not a.no_extra:
-        st = None  # release the 2^30 state before the batch workloads
         print("bench: gradient workload", file=sys.stderr, flush=True)
         extra["ps_gradient_16q"] = ps_gradient_bench(lq, cpu=not a.no_cpu)
         print("bench: uneven batch workload", file=sys.stderr, flush=True)
@@ -266,7 +314,6 @@ def main():
     out = {
         "metric": "gate HBM GB/s at 30q c128; VQE time-to-dE<1e-4, warm-start+batched vs plain",
         "value": round(value, 2),
-        "effective_gate_GBps": round(bytes_gates / step_s / 1e9, 2),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -279,13 +326,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "random brickwork circuit, %d qubits, depth %d, %s, seed %d, %s execution"
                                % (a.qubits, a.depth, a.dtype, a.seed, a.mode.replace("_", "-")),
-                   "qubits": a.qubits, "depth": a.depth, "gates": len(descs),
-                   "passes": passes, "autotune": tune, "hbm_bytes_per_step": bytes_moved,
-                   "gate_algorithmic_bytes_per_step": bytes_gates, "final_norm_sqr": norm2,
+                   "qubits": a.qubits, "depth": a.depth, "gates": len(descs), "autotune": tune,
+                   "hbm_bytes_per_step": bytes_moved, "gate_algorithmic_bytes_per_step": bytes_gates,
+                   "final_norm_sqr": norm2,
                    "per_gate_kernels": {"rx_pair_GBps": round(rx_gbs, 1), "rx_frac": round(rx_gbs / HBM_PEAK_GBS, 4),
-                                        "cnot_GBps": round(cn_gbs, 1), "cnot_frac": round(cn_gbs / HBM_PEAK_GBS, 4)}},
+                                        "cnot_GBps": round(cn_gbs, 1), "cnot_frac": round(cn_gbs / HBM_PEAK_GBS, 4),
+                                        "c64": c64},
+                   "fused_execution": fused},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.mode),
                      "kernel": "%s, %d B per launch, avg %.3f ms" % (kname, bpl, avg_s * 1e3)},
         "cpu_baseline": cpu,
         "workloads": extra,

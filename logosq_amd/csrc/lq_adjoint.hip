@@ -101,6 +101,18 @@ struct AdjMeta {
     int32_t gbit;   // Z: global bit of the generator
 };
 
+// A run of consecutive ops of a backward pass applied in one sweep over the
+// tile: every 2x2 op of the run acts on local bit lb (diagonal ops may act
+// anywhere), and at most kRunParams of them carry a parameter.  lb = -1: a
+// single SWAP / 4x4 op, applied op by op.
+constexpr int kRunParams = 4;
+struct AdjRun {
+    int32_t first;  // first op, relative to the pass
+    int32_t count;
+    int32_t lb;
+    int32_t pad;
+};
+
 __device__ __forceinline__ uint32_t ins0a(uint32_t k, int p) {
     return ((k >> p) << (p + 1)) | (k & ((1u << p) - 1u));
 }
@@ -205,7 +217,8 @@ __device__ __forceinline__ void adj_apply(V* tile, const TileOp& op, const G& gi
 template <int K, typename V, typename T>
 __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* __restrict__ lam, TilePass p,
                                                         const TileOp* __restrict__ ops,
-                                                        const AdjMeta* __restrict__ meta, uint64_t ninst,
+                                                        const AdjMeta* __restrict__ meta,
+                                                        const AdjRun* __restrict__ runs, int nruns, uint64_t ninst,
                                                         uint64_t inst_stride, uint64_t ops_stride,
                                                         double* __restrict__ partial, int nslots, uint64_t nwp) {
     constexpr uint32_t TS = 1u << K;
@@ -256,49 +269,128 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
             }
         }
         __syncthreads();
-        for (int o = 0; o < p.op_count; ++o) {
-            const int slot = md[o].slot;
-            if (slot >= 0) {
-                // this tile's share of Im <lambda| P |psi>, before the inverse
-                const int pauli = md[o].pauli;
-                double acc = 0.0;
-                if (pauli == LQ_PAULI_Z) {
-                    const int gb = md[o].gbit;
+        for (int r = 0; r < nruns; ++r) {
+            const int rfirst = runs[r].first, rcount = runs[r].count, lb = runs[r].lb;
+            if (lb < 0) {
+                adj_apply<K, V, T>(ta, op[rfirst], gidx);
+                adj_apply<K, V, T>(tb, op[rfirst], gidx);
+                __syncthreads();
+                continue;
+            }
+            const uint64_t gmask = 1ull << (lb < L ? lb : p.hpos[lb - L]);
+            double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
 #pragma unroll
-                    for (int j = 0; j < EPT; ++j) {
-                        const uint32_t e = tid + (uint32_t)j * NT;
-                        if (!own) continue;
-                        const V l = tb[e], s = ta[e];
-                        const double v = (double)l.x * (double)s.y - (double)l.y * (double)s.x;
-                        acc += ((gidx(e) >> gb) & 1ull) ? -v : v;
-                    }
-                } else {
-                    const int lb = md[o].lb;
-#pragma unroll
-                    for (int j = 0; j < PPT; ++j) {
-                        const uint32_t q = tid + (uint32_t)j * NT;
-                        if (!PFULL && q >= TS / 2) continue;
-                        const uint32_t e0 = ins0a(q, lb);
-                        const uint32_t e1 = e0 | (1u << lb);
-                        const V l0 = tb[e0], l1 = tb[e1], s0 = ta[e0], s1 = ta[e1];
-                        if (pauli == LQ_PAULI_X) {
+            for (int j = 0; j < PPT; ++j) {
+                const uint32_t q = tid + (uint32_t)j * NT;
+                if (!PFULL && q >= TS / 2) continue;
+                const uint32_t e0 = ins0a(q, lb);
+                const uint32_t e1 = e0 | (1u << lb);
+                const uint64_t g0 = gidx(e0), g1 = g0 | gmask;
+                V a0 = ta[e0], a1 = ta[e1], b0 = tb[e0], b1 = tb[e1];
+                int k = 0;
+                for (int o = rfirst; o < rfirst + rcount; ++o) {
+                    const TileOp& po = op[o];
+                    if (md[o].slot >= 0) {
+                        // share of Im <lambda| P |psi> right before the inverse
+                        const int pauli = md[o].pauli;
+                        double c;
+                        if (pauli == LQ_PAULI_Z) {
+                            const int gb = md[o].gbit;
+                            const double v0 = (double)b0.x * a0.y - (double)b0.y * a0.x;
+                            const double v1 = (double)b1.x * a1.y - (double)b1.y * a1.x;
+                            c = (((g0 >> gb) & 1ull) ? -v0 : v0) + (((g1 >> gb) & 1ull) ? -v1 : v1);
+                        } else if (pauli == LQ_PAULI_X) {
                             // (X psi)_e0 = psi_e1, (X psi)_e1 = psi_e0
-                            acc += ((double)l0.x * s1.y - (double)l0.y * s1.x) +
-                                   ((double)l1.x * s0.y - (double)l1.y * s0.x);
+                            c = ((double)b0.x * a1.y - (double)b0.y * a1.x) + ((double)b1.x * a0.y - (double)b1.y * a0.x);
                         } else {
                             // (Y psi)_e0 = -i psi_e1, (Y psi)_e1 = i psi_e0
-                            acc += -((double)l0.x * s1.x + (double)l0.y * s1.y) +
-                                   ((double)l1.x * s0.x + (double)l1.y * s0.y);
+                            c = -((double)b0.x * a1.x + (double)b0.y * a1.y) + ((double)b1.x * a0.x + (double)b1.y * a0.y);
+                        }
+                        acc0 += k == 0 ? c : 0.0;
+                        acc1 += k == 1 ? c : 0.0;
+                        acc2 += k == 2 ? c : 0.0;
+                        acc3 += k == 3 ? c : 0.0;
+                        ++k;
+                    }
+                    if (po.kind == kTDiag) {
+                        const uint64_t ctrl = po.ctrl;
+                        const int s0 = po.sel0, s1 = po.sel1;
+                        if ((g0 & ctrl) == ctrl) {
+                            const int i0 = (s0 >= 0 ? (int)((g0 >> s0) & 1ull) : 0) |
+                                           ((s1 >= 0 ? (int)((g0 >> s1) & 1ull) : 0) << 1);
+                            const T pr = (T)po.m[2 * i0], pi = (T)po.m[2 * i0 + 1];
+                            a0 = cmul(pr, pi, a0);
+                            b0 = cmul(pr, pi, b0);
+                        }
+                        if ((g1 & ctrl) == ctrl) {
+                            const int i1 = (s0 >= 0 ? (int)((g1 >> s0) & 1ull) : 0) |
+                                           ((s1 >= 0 ? (int)((g1 >> s1) & 1ull) : 0) << 1);
+                            const T pr = (T)po.m[2 * i1], pi = (T)po.m[2 * i1 + 1];
+                            a1 = cmul(pr, pi, a1);
+                            b1 = cmul(pr, pi, b1);
+                        }
+                    } else if (!po.ctrl || (g0 & po.ctrl) == po.ctrl) {
+                        // 2x2 on bit lb (controls never include lb)
+                        const int kind = po.kind;
+                        if (kind == kTPairMat) {
+                            const T m0 = (T)po.m[0], m1 = (T)po.m[1], m2 = (T)po.m[2], m3 = (T)po.m[3];
+                            const T m4 = (T)po.m[4], m5 = (T)po.m[5], m6 = (T)po.m[6], m7 = (T)po.m[7];
+                            V x0 = cadd(cmul(m0, m1, a0), cmul(m2, m3, a1));
+                            V x1 = cadd(cmul(m4, m5, a0), cmul(m6, m7, a1));
+                            V y0 = cadd(cmul(m0, m1, b0), cmul(m2, m3, b1));
+                            V y1 = cadd(cmul(m4, m5, b0), cmul(m6, m7, b1));
+                            a0 = x0;
+                            a1 = x1;
+                            b0 = y0;
+                            b1 = y1;
+                        } else if (kind == kTPairSwap) {
+                            V t0 = a0;
+                            a0 = a1;
+                            a1 = t0;
+                            t0 = b0;
+                            b0 = b1;
+                            b1 = t0;
+                        } else if (kind == kTPairNegY) {
+                            V x0 = cmul((T)0, (T)1, a1), x1 = cmul((T)0, (T)-1, a0);
+                            V y0 = cmul((T)0, (T)1, b1), y1 = cmul((T)0, (T)-1, b0);
+                            a0 = x0;
+                            a1 = x1;
+                            b0 = y0;
+                            b1 = y1;
+                        } else {  // kTPairH
+                            const T h = (T)po.m[0];
+                            V sa0, da0, sb0, db0;
+                            sa0.x = a0.x + a1.x;
+                            sa0.y = a0.y + a1.y;
+                            da0.x = a0.x - a1.x;
+                            da0.y = a0.y - a1.y;
+                            sb0.x = b0.x + b1.x;
+                            sb0.y = b0.y + b1.y;
+                            db0.x = b0.x - b1.x;
+                            db0.y = b0.y - b1.y;
+                            a0 = cmul(h, (T)0, sa0);
+                            a1 = cmul(h, (T)0, da0);
+                            b0 = cmul(h, (T)0, sb0);
+                            b1 = cmul(h, (T)0, db0);
                         }
                     }
                 }
+                ta[e0] = a0;
+                ta[e1] = a1;
+                tb[e0] = b0;
+                tb[e1] = b1;
+            }
+            // the run's gradient shares: one partial per wavefront
+            int k = 0;
+            for (int o = rfirst; o < rfirst + rcount; ++o) {
+                const int slot = md[o].slot;
+                if (slot < 0) continue;
+                double acc = k == 0 ? acc0 : k == 1 ? acc1 : k == 2 ? acc2 : acc3;
+                ++k;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
                 if (lane == 0) partial[((uint64_t)inst * nslots + slot) * nwp + t * kAWaves + wave] = acc;
-                __syncthreads();
             }
-            adj_apply<K, V, T>(ta, op[o], gidx);
-            adj_apply<K, V, T>(tb, op[o], gidx);
             __syncthreads();
         }
         if (own) {
@@ -337,7 +429,7 @@ __global__ void __launch_bounds__(kAThreads) k_rows_sum(const double* __restrict
 
 template <int K, typename V, typename T>
 static void launch_adj(void* psi, void* lam, const TilePass& p, const TileOp* ops, const AdjMeta* meta,
-                       uint64_t ninst, uint64_t stride, uint64_t ostride, double* partial, int nslots, uint64_t nwp,
+                       const AdjRun* runs, int nruns, uint64_t ninst, uint64_t stride, uint64_t ostride, double* partial, int nslots, uint64_t nwp,
                        unsigned grid, size_t lds) {
     static std::once_flag once;
     std::call_once(once, [&] {
@@ -345,18 +437,19 @@ static void launch_adj(void* psi, void* lam, const TilePass& p, const TileOp* op
                                   (int)lds + 4096);
     });
     hipLaunchKernelGGL((k_tile_adj<K, V, T>), dim3(grid), dim3(kAThreads), lds, stream(), (V*)psi, (V*)lam, p, ops,
-                       meta, ninst, stride, ostride, partial, nslots, nwp);
+                       meta, runs, nruns, ninst, stride, ostride, partial, nslots, nwp);
 }
 
 template <typename V, typename T>
 static int dispatch_adj(int K, void* psi, void* lam, const TilePass& p, const TileOp* ops, const AdjMeta* meta,
-                        uint64_t ninst, uint64_t stride, uint64_t ostride, double* partial, int nslots, uint64_t nwp) {
+                        const AdjRun* runs, int nruns, uint64_t ninst, uint64_t stride, uint64_t ostride, double* partial, int nslots, uint64_t nwp) {
     const size_t lds = 2 * (sizeof(V) << K) + (sizeof(uint64_t) << p.hb);
     const unsigned grid = (unsigned)std::min<uint64_t>(ninst * p.ntiles, 8192);
     switch (K) {
 #define LQ_KA(k)                                                                                             \
     case k:                                                                                                  \
-        launch_adj<k, V, T>(psi, lam, p, ops, meta, ninst, stride, ostride, partial, nslots, nwp, grid, lds); \
+        launch_adj<k, V, T>(psi, lam, p, ops, meta, runs, nruns, ninst, stride, ostride, partial, nslots, nwp, \
+                            grid, lds);                                                                     \
         break;
         LQ_KA(1) LQ_KA(2) LQ_KA(3) LQ_KA(4) LQ_KA(5) LQ_KA(6) LQ_KA(7) LQ_KA(8) LQ_KA(9) LQ_KA(10) LQ_KA(11)
         LQ_KA(12)
@@ -542,6 +635,49 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
         }
         meta[o] = m;
     }
+    // runs of the backward passes (see AdjRun)
+    std::vector<AdjRun> runs;
+    std::vector<std::pair<int, int>> pass_runs;  // per backward step: first run, run count
+    for (const Step& st : plb.steps) {
+        const int r0 = (int)runs.size();
+        if (!st.normalize) {
+            AdjRun cur{0, 0, -1, 0};
+            int cur_params = 0;
+            bool open = false;
+            for (int o = 0; o < st.pass.op_count; ++o) {
+                const TileOp& op = plb.ops[st.pass.op_first + o];
+                const int par = meta[st.pass.op_first + o].slot >= 0 ? 1 : 0;
+                if (op.kind == kTSwap2 || op.kind == kTQuad) {
+                    if (open) {
+                        if (cur.lb < 0) cur.lb = 0;
+                        runs.push_back(cur);
+                        open = false;
+                    }
+                    runs.push_back(AdjRun{o, 1, -1, 0});
+                    continue;
+                }
+                const bool pair = op.kind != kTDiag;
+                if (open && (!pair || cur.lb < 0 || cur.lb == op.lb0) && cur_params + par <= kRunParams) {
+                    cur.count++;
+                    cur_params += par;
+                    if (pair) cur.lb = op.lb0;
+                    continue;
+                }
+                if (open) {
+                    if (cur.lb < 0) cur.lb = 0;  // diagonal ops only: any pairing bit
+                    runs.push_back(cur);
+                }
+                cur = AdjRun{o, 1, pair ? op.lb0 : -1, 0};
+                cur_params = par;
+                open = true;
+            }
+            if (open) {
+                if (cur.lb < 0) cur.lb = 0;
+                runs.push_back(cur);
+            }
+        }
+        pass_runs.emplace_back(r0, (int)runs.size() - r0);
+    }
     for (const Step& st : plf.steps)
         if (st.normalize) {
             set_error("adjoint gradient: renormalising step in the forward plan");
@@ -569,7 +705,7 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
         const int cn = std::min(chunk, count - c0);
         const size_t b_of = al(cn * nof * sizeof(TileOp) + 1), b_ob = al(cn * nob * sizeof(TileOp) + 1);
         const size_t b_gr = al(plf.groups.size() * sizeof(TileGroup) + 1);
-        const size_t b_me = al(meta.size() * sizeof(AdjMeta) + 1);
+        const size_t b_me = al(meta.size() * sizeof(AdjMeta) + 1) + al(runs.size() * sizeof(AdjRun) + 1);
         const size_t b_te = al((size_t)cn * std::max(nt, 1) * sizeof(PauliTermDev));
         const size_t b_pa = al((size_t)cn * std::max(nslots, 1) * nwp * sizeof(double));
         const size_t b_pe = al((size_t)cn * nbE * sizeof(double));
@@ -617,11 +753,14 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
         if (!plf.groups.empty())
             std::memcpy(hb + b_of + b_ob, plf.groups.data(), plf.groups.size() * sizeof(TileGroup));
         if (!meta.empty()) std::memcpy(hb + b_of + b_ob + b_gr, meta.data(), meta.size() * sizeof(AdjMeta));
+        const size_t runs_off = b_of + b_ob + b_gr + al(meta.size() * sizeof(AdjMeta) + 1);
+        if (!runs.empty()) std::memcpy(hb + runs_off, runs.data(), runs.size() * sizeof(AdjRun));
         LQ_HIP_CHECK(hipMemcpyAsync(base, hb, up, hipMemcpyHostToDevice, stream()));
         const TileOp* d_of = (const TileOp*)base;
         const TileOp* d_ob = (const TileOp*)(base + b_of);
         const TileGroup* d_gr = (const TileGroup*)(base + b_of + b_ob);
         const AdjMeta* d_me = (const AdjMeta*)(base + b_of + b_ob + b_gr);
+        const AdjRun* d_ru = (const AdjRun*)(base + runs_off);
         const PauliTermDev* d_te = (const PauliTermDev*)(base + b_of + b_ob + b_gr + b_me);
         double* d_pa = (double*)(base + up);
         double* d_pe = (double*)(base + up + b_pa);
@@ -651,8 +790,11 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
             LQ_LAUNCH_CHECK();
         }
         // backward sweep
-        for (const Step& st : plb.steps) {
-            rc = dispatch_adj<V, T>(Kb, psi, lam, st.pass, d_ob, d_me, cn, dim, nob, d_pa, nslots, nwp);
+        for (size_t si = 0; si < plb.steps.size(); ++si) {
+            const Step& st = plb.steps[si];
+            if (st.normalize) continue;
+            rc = dispatch_adj<V, T>(Kb, psi, lam, st.pass, d_ob, d_me, d_ru + pass_runs[si].first,
+                                    pass_runs[si].second, cn, dim, nob, d_pa, nslots, nwp);
             if (rc) return rc;
         }
         if (nslots > 0) {

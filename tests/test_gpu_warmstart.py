@@ -1,0 +1,45 @@
+"""The learned warm start against the plain initialisation on held-out
+couplings (XYZ rings, RY/RZ + CZ-ring ansatz): the network, trained on the
+GPU on converged VQE solutions this simulator generates, must start the held-
+out VQEs lower and reach within 1e-4 of the converged energy in fewer Adam
+iterations.  Each method gets its best learning rate from a small grid,
+chosen on validation couplings that are neither trained on nor held out."""
+import numpy as np
+import pytest
+
+import logosq_amd as lq
+from logosq_amd import warmstart
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warm_start_gains_over_plain(hip):
+    n, layers = 8, 2
+    ans = lq.ry_rz_cz_ring_ansatz(n, layers)
+    P = ans.num_parameters()
+    theta0 = warmstart.plain_init(P, 11)
+    Jtr = warmstart.couplings(256, 21)
+    Jval = warmstart.couplings(16, 23)
+    Jte = warmstart.couplings(24, 22)
+    print("warm start: generating %d training solutions" % len(Jtr), flush=True)
+    solved, X, E, it = warmstart.generate_by_continuation(ans, Jtr, theta0,
+                                                          lq.Adam(0.05, 400).with_tolerance(1e-8), batch=64)
+    assert solved.all() and np.all(np.isfinite(E))
+    print("warm start: training", flush=True)
+    ws, hist = warmstart.WarmStart.train(Jtr, X, theta0, hidden=128, epochs=400, batch=64, learning_rate=2e-3)
+    print("loss %.4g -> %.4g" % (hist[0], hist[-1]), flush=True)
+    assert hist[-1] < 0.2 * hist[0]
+    grid, cap, tol = (0.01, 0.02, 0.05, 0.1), 800, 1e-10
+    lr_p, med_p = warmstart.pick_learning_rate(ans, Jval, np.tile(theta0, (len(Jval), 1)), grid, cap, tol)
+    lr_w, med_w = warmstart.pick_learning_rate(ans, Jval, warmstart.choose_starts(ws, ans, Jval)[0], grid, cap, tol)
+    print("learning rates: plain %g %s warm %g %s" % (lr_p, med_p, lr_w, med_w), flush=True)
+    rows = warmstart.evaluate(ans, ws, Jte, theta0, lr_p, lr_w, cap, tol)
+    used = sum(r["used_proposal"] for r in rows)
+    ip = np.array([r["iters_plain"] or cap for r in rows])
+    iw = np.array([r["iters_warm"] or cap for r in rows])
+    print("proposals used %d of %d; iterations to dE<1e-4: plain %s warm %s" % (used, len(rows), ip.tolist(),
+                                                                                  iw.tolist()), flush=True)
+    assert used >= 0.75 * len(rows)
+    assert np.mean([r["e_start_proposal"] for r in rows]) < np.mean([r["e_start_plain"] for r in rows])
+    assert np.median(iw) < 0.5 * np.median(ip)
+    assert iw.sum() < 0.5 * ip.sum()
