@@ -279,109 +279,137 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
             }
             const uint64_t gmask = 1ull << (lb < L ? lb : p.hpos[lb - L]);
             double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+            // this thread's pairs (e0, e1) along bit lb, of both tiles, in registers
+            V a0[PPT], a1[PPT], b0[PPT], b1[PPT];
+            uint64_t g0[PPT];
+            bool live[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const uint32_t q = tid + (uint32_t)j * NT;
-                if (!PFULL && q >= TS / 2) continue;
-                const uint32_t e0 = ins0a(q, lb);
+                live[j] = PFULL || q < TS / 2;
+                const uint32_t e0 = ins0a(live[j] ? q : 0u, lb);
                 const uint32_t e1 = e0 | (1u << lb);
-                const uint64_t g0 = gidx(e0), g1 = g0 | gmask;
-                V a0 = ta[e0], a1 = ta[e1], b0 = tb[e0], b1 = tb[e1];
-                int k = 0;
-                for (int o = rfirst; o < rfirst + rcount; ++o) {
-                    const TileOp& po = op[o];
-                    if (md[o].slot >= 0) {
-                        // share of Im <lambda| P |psi> right before the inverse
-                        const int pauli = md[o].pauli;
-                        double c;
+                g0[j] = gidx(e0);
+                a0[j] = ta[e0];
+                a1[j] = ta[e1];
+                b0[j] = tb[e0];
+                b1[j] = tb[e1];
+            }
+            int k = 0;
+            for (int o = rfirst; o < rfirst + rcount; ++o) {
+                const TileOp& po = op[o];
+                if (md[o].slot >= 0) {
+                    // share of Im <lambda| P |psi> right before the inverse
+                    const int pauli = md[o].pauli;
+                    const int gb = md[o].gbit;
+                    double c = 0.0;
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        double cj;
                         if (pauli == LQ_PAULI_Z) {
-                            const int gb = md[o].gbit;
-                            const double v0 = (double)b0.x * a0.y - (double)b0.y * a0.x;
-                            const double v1 = (double)b1.x * a1.y - (double)b1.y * a1.x;
-                            c = (((g0 >> gb) & 1ull) ? -v0 : v0) + (((g1 >> gb) & 1ull) ? -v1 : v1);
+                            const double v0 = (double)b0[j].x * a0[j].y - (double)b0[j].y * a0[j].x;
+                            const double v1 = (double)b1[j].x * a1[j].y - (double)b1[j].y * a1[j].x;
+                            const uint64_t g1 = g0[j] | gmask;
+                            cj = (((g0[j] >> gb) & 1ull) ? -v0 : v0) + (((g1 >> gb) & 1ull) ? -v1 : v1);
                         } else if (pauli == LQ_PAULI_X) {
                             // (X psi)_e0 = psi_e1, (X psi)_e1 = psi_e0
-                            c = ((double)b0.x * a1.y - (double)b0.y * a1.x) + ((double)b1.x * a0.y - (double)b1.y * a0.x);
+                            cj = ((double)b0[j].x * a1[j].y - (double)b0[j].y * a1[j].x) +
+                                 ((double)b1[j].x * a0[j].y - (double)b1[j].y * a0[j].x);
                         } else {
                             // (Y psi)_e0 = -i psi_e1, (Y psi)_e1 = i psi_e0
-                            c = -((double)b0.x * a1.x + (double)b0.y * a1.y) + ((double)b1.x * a0.x + (double)b1.y * a0.y);
+                            cj = -((double)b0[j].x * a1[j].x + (double)b0[j].y * a1[j].y) +
+                                 ((double)b1[j].x * a0[j].x + (double)b1[j].y * a0[j].y);
                         }
-                        acc0 += k == 0 ? c : 0.0;
-                        acc1 += k == 1 ? c : 0.0;
-                        acc2 += k == 2 ? c : 0.0;
-                        acc3 += k == 3 ? c : 0.0;
-                        ++k;
+                        c += live[j] ? cj : 0.0;
                     }
-                    if (po.kind == kTDiag) {
-                        const uint64_t ctrl = po.ctrl;
-                        const int s0 = po.sel0, s1 = po.sel1;
-                        if ((g0 & ctrl) == ctrl) {
-                            const int i0 = (s0 >= 0 ? (int)((g0 >> s0) & 1ull) : 0) |
-                                           ((s1 >= 0 ? (int)((g0 >> s1) & 1ull) : 0) << 1);
-                            const T pr = (T)po.m[2 * i0], pi = (T)po.m[2 * i0 + 1];
-                            a0 = cmul(pr, pi, a0);
-                            b0 = cmul(pr, pi, b0);
+                    acc0 += k == 0 ? c : 0.0;
+                    acc1 += k == 1 ? c : 0.0;
+                    acc2 += k == 2 ? c : 0.0;
+                    acc3 += k == 3 ? c : 0.0;
+                    ++k;
+                }
+                const int kind = po.kind;
+                const uint64_t ctrl = po.ctrl;
+                if (kind == kTDiag) {
+                    const int s0 = po.sel0, s1 = po.sel1;
+                    const T r0 = (T)po.m[0], i0 = (T)po.m[1], r1 = (T)po.m[2], i1 = (T)po.m[3];
+                    const T r2 = (T)po.m[4], i2 = (T)po.m[5], r3 = (T)po.m[6], i3 = (T)po.m[7];
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const uint64_t g = h ? (g0[j] | gmask) : g0[j];
+                            const bool on = (g & ctrl) == ctrl;
+                            const bool c0 = s0 >= 0 && ((g >> s0) & 1ull);
+                            const bool c1 = s1 >= 0 && ((g >> s1) & 1ull);
+                            const T pr = on ? (c1 ? (c0 ? r3 : r2) : (c0 ? r1 : r0)) : (T)1;
+                            const T pi = on ? (c1 ? (c0 ? i3 : i2) : (c0 ? i1 : i0)) : (T)0;
+                            if (h) {
+                                a1[j] = cmul(pr, pi, a1[j]);
+                                b1[j] = cmul(pr, pi, b1[j]);
+                            } else {
+                                a0[j] = cmul(pr, pi, a0[j]);
+                                b0[j] = cmul(pr, pi, b0[j]);
+                            }
                         }
-                        if ((g1 & ctrl) == ctrl) {
-                            const int i1 = (s0 >= 0 ? (int)((g1 >> s0) & 1ull) : 0) |
-                                           ((s1 >= 0 ? (int)((g1 >> s1) & 1ull) : 0) << 1);
-                            const T pr = (T)po.m[2 * i1], pi = (T)po.m[2 * i1 + 1];
-                            a1 = cmul(pr, pi, a1);
-                            b1 = cmul(pr, pi, b1);
-                        }
-                    } else if (!po.ctrl || (g0 & po.ctrl) == po.ctrl) {
-                        // 2x2 on bit lb (controls never include lb)
-                        const int kind = po.kind;
+                    }
+                } else {
+                    // 2x2 on bit lb (controls never include lb)
+                    const T m0 = (T)po.m[0], m1 = (T)po.m[1], m2 = (T)po.m[2], m3 = (T)po.m[3];
+                    const T m4 = (T)po.m[4], m5 = (T)po.m[5], m6 = (T)po.m[6], m7 = (T)po.m[7];
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        if (ctrl && (g0[j] & ctrl) != ctrl) continue;
+                        V x0, x1, y0, y1;
                         if (kind == kTPairMat) {
-                            const T m0 = (T)po.m[0], m1 = (T)po.m[1], m2 = (T)po.m[2], m3 = (T)po.m[3];
-                            const T m4 = (T)po.m[4], m5 = (T)po.m[5], m6 = (T)po.m[6], m7 = (T)po.m[7];
-                            V x0 = cadd(cmul(m0, m1, a0), cmul(m2, m3, a1));
-                            V x1 = cadd(cmul(m4, m5, a0), cmul(m6, m7, a1));
-                            V y0 = cadd(cmul(m0, m1, b0), cmul(m2, m3, b1));
-                            V y1 = cadd(cmul(m4, m5, b0), cmul(m6, m7, b1));
-                            a0 = x0;
-                            a1 = x1;
-                            b0 = y0;
-                            b1 = y1;
+                            x0 = cadd(cmul(m0, m1, a0[j]), cmul(m2, m3, a1[j]));
+                            x1 = cadd(cmul(m4, m5, a0[j]), cmul(m6, m7, a1[j]));
+                            y0 = cadd(cmul(m0, m1, b0[j]), cmul(m2, m3, b1[j]));
+                            y1 = cadd(cmul(m4, m5, b0[j]), cmul(m6, m7, b1[j]));
                         } else if (kind == kTPairSwap) {
-                            V t0 = a0;
-                            a0 = a1;
-                            a1 = t0;
-                            t0 = b0;
-                            b0 = b1;
-                            b1 = t0;
+                            x0 = a1[j];
+                            x1 = a0[j];
+                            y0 = b1[j];
+                            y1 = b0[j];
                         } else if (kind == kTPairNegY) {
-                            V x0 = cmul((T)0, (T)1, a1), x1 = cmul((T)0, (T)-1, a0);
-                            V y0 = cmul((T)0, (T)1, b1), y1 = cmul((T)0, (T)-1, b0);
-                            a0 = x0;
-                            a1 = x1;
-                            b0 = y0;
-                            b1 = y1;
-                        } else {  // kTPairH
-                            const T h = (T)po.m[0];
-                            V sa0, da0, sb0, db0;
-                            sa0.x = a0.x + a1.x;
-                            sa0.y = a0.y + a1.y;
-                            da0.x = a0.x - a1.x;
-                            da0.y = a0.y - a1.y;
-                            sb0.x = b0.x + b1.x;
-                            sb0.y = b0.y + b1.y;
-                            db0.x = b0.x - b1.x;
-                            db0.y = b0.y - b1.y;
-                            a0 = cmul(h, (T)0, sa0);
-                            a1 = cmul(h, (T)0, da0);
-                            b0 = cmul(h, (T)0, sb0);
-                            b1 = cmul(h, (T)0, db0);
+                            x0 = cmul((T)0, (T)1, a1[j]);
+                            x1 = cmul((T)0, (T)-1, a0[j]);
+                            y0 = cmul((T)0, (T)1, b1[j]);
+                            y1 = cmul((T)0, (T)-1, b0[j]);
+                        } else {  // kTPairH: m0 = 1/sqrt(2)
+                            V sa, da, sb, db;
+                            sa.x = a0[j].x + a1[j].x;
+                            sa.y = a0[j].y + a1[j].y;
+                            da.x = a0[j].x - a1[j].x;
+                            da.y = a0[j].y - a1[j].y;
+                            sb.x = b0[j].x + b1[j].x;
+                            sb.y = b0[j].y + b1[j].y;
+                            db.x = b0[j].x - b1[j].x;
+                            db.y = b0[j].y - b1[j].y;
+                            x0 = cmul(m0, (T)0, sa);
+                            x1 = cmul(m0, (T)0, da);
+                            y0 = cmul(m0, (T)0, sb);
+                            y1 = cmul(m0, (T)0, db);
                         }
+                        a0[j] = x0;
+                        a1[j] = x1;
+                        b0[j] = y0;
+                        b1[j] = y1;
                     }
                 }
-                ta[e0] = a0;
-                ta[e1] = a1;
-                tb[e0] = b0;
-                tb[e1] = b1;
+            }
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                if (!live[j]) continue;
+                const uint32_t e0 = ins0a(tid + (uint32_t)j * NT, lb);
+                const uint32_t e1 = e0 | (1u << lb);
+                ta[e0] = a0[j];
+                ta[e1] = a1[j];
+                tb[e0] = b0[j];
+                tb[e1] = b1[j];
             }
             // the run's gradient shares: one partial per wavefront
-            int k = 0;
+            k = 0;
             for (int o = rfirst; o < rfirst + rcount; ++o) {
                 const int slot = md[o].slot;
                 if (slot < 0) continue;
@@ -462,8 +490,13 @@ static int dispatch_adj(int K, void* psi, void* lam, const TilePass& p, const Ti
     return LQ_OK;
 }
 
-// Backward-sweep tiles: two tiles (psi, lambda) of 32 KiB each in LDS.
-static int adj_tile_bits(int n, int dtype) { return std::min(n, dtype == LQ_C128 ? 11 : 12); }
+// Backward-sweep tiles: 2^11 amplitudes of psi and of lambda in LDS (64 KiB
+// for complex128, 32 KiB for complex64); each thread holds 4 pairs of each
+// in registers through a run.
+static int adj_tile_bits(int n, int dtype) {
+    (void)dtype;
+    return std::min(n, 11);
+}
 
 // Device buffers of the gradient (grown on demand, reused): the backward
 // plan's ops and metadata, Pauli terms, partials and results.  Dedicated

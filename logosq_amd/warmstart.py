@@ -287,7 +287,7 @@ def cmd_generate(a):
         state = (solved, X, E, it)
         print("resuming: %d of %d solved" % (solved.sum(), len(J)), flush=True)
     opt = Adam(w["learning_rate"], w["max_iterations"]).with_tolerance(w["tolerance"])
-    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, log=lambda m: print(m, flush=True),
+    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, seeds=1, log=lambda m: print(m, flush=True),
                                      state=state, time_limit=a.time_limit,
                                      checkpoint=lambda s: _save(a.out, s, J, theta0, w))
     print("done: %d of %d solved" % (state[0].sum(), len(J)), flush=True)
@@ -359,6 +359,7 @@ def cmd_evaluate(a):
     rows = evaluate(ans, ws, Jte, theta0, lr_p, lr_w, a.max_iterations, a.tolerance)
     with open(a.out, "w") as f:
         json.dump({"meta": meta, "train_loss": [float(v) for v in hist], "lr_plain": lr_p, "lr_warm": lr_w,
+                   "epochs": a.epochs, "max_iterations": a.max_iterations, "tolerance": a.tolerance,
                    "validation_median_iterations": {"plain": med_p, "warm": med_w}, "rows": rows}, f)
     ip = [r["iters_plain"] for r in rows]
     iw = [r["iters_warm"] for r in rows]
@@ -384,8 +385,8 @@ def main(argv=None):
     e.add_argument("--lr", type=float, default=1e-3)
     e.add_argument("--max-iterations", type=int, default=1500)
     e.add_argument("--tolerance", type=float, default=1e-10)
-    e.add_argument("--validation", type=int, default=32)
-    e.add_argument("--lr-grid", default="0.01,0.02,0.05,0.1")
+    e.add_argument("--validation", type=int, default=16)
+    e.add_argument("--lr-grid", default="0.02,0.05,0.1")
     a = ap.parse_args(argv)
     {"generate": cmd_generate, "evaluate": cmd_evaluate}[a.cmd](a)
 
