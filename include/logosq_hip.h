@@ -258,6 +258,14 @@ int lq_batch_expectations(int count, const int* num_qubits, const lq_gate* const
                           const uint8_t* const* paulis, const double* const* coeffs, const int* num_terms,
                           int convention, int dtype, double* energies);
 
+/* Batches (parameter-shift gradients and lq_batch_expectations) run their
+ * device work as one launch sequence per chunk; with graphs on (default) a
+ * sequence shape seen a second time is captured into a hipGraph and replayed
+ * from then on (no reference counterpart).  Results are identical either way. */
+int lq_set_batch_graphs(int enable);
+/* Graph captures and replays so far in this process. */
+int lq_batch_graph_stats(long long* captures, long long* replays);
+
 /* ---- Optimizers and VQE (src/optimization/optimizer.rs, vqe.rs) -------- */
 typedef double (*lq_objective_fn)(const double* x, int dim, void* user);
 typedef void (*lq_gradient_fn)(const double* x, int dim, double* grad, void* user);

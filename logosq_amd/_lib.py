@@ -67,6 +67,8 @@ SIGNATURES = {
                                   P_double, P_double],
     "lq_vqe_adam_batch": [c_int, c_int, P_gate, c_int, c_int, P_u8, P_double, c_int, c_int, c_int, c_double, c_int,
                           c_double, P_double, P_double, P_double, P_int, P_int],
+    "lq_set_batch_graphs": [c_int],
+    "lq_batch_graph_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "lq_batch_expectations": [c_int, P_int, ctypes.POINTER(P_gate), P_int, ctypes.POINTER(P_u8),
                               ctypes.POINTER(P_double), P_int, c_int, c_int, P_double],
     "lq_gd_minimize": [c_double, c_int, c_double, OBJ_FN, GRAD_FN, c_void_p, P_double, c_int, P_double, c_void_p],

@@ -212,6 +212,37 @@ __device__ __forceinline__ void adj_apply(V* tile, const TileOp& op, const G& gi
     }
 }
 
+// on ? x : y, component by component (a select of the whole struct would
+// keep the arrays it lands in out of registers)
+template <typename V>
+__device__ __forceinline__ V vsel(bool on, V x, V y) {
+    V r;
+    r.x = on ? x.x : y.x;
+    r.y = on ? x.y : y.y;
+    return r;
+}
+
+// The fields of an op (and its gradient bookkeeping) that a run reads.
+struct OpLite {
+    int kind, sel0, sel1, slot, pauli, gbit;
+    uint64_t ctrl;
+    double m[8];
+};
+
+__device__ __forceinline__ OpLite op_lite(const TileOp& t, const AdjMeta& md) {
+    OpLite r;
+    r.kind = t.kind;
+    r.sel0 = t.sel0;
+    r.sel1 = t.sel1;
+    r.ctrl = t.ctrl;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r.m[i] = t.m[i];
+    r.slot = md.slot;
+    r.pauli = md.pauli;
+    r.gbit = md.gbit;
+    return r;
+}
+
 // Backward tile pass over `ninst` (psi, lambda) pairs.  partial layout:
 // [inst][slot][tile][wave] (nwp = tiles * waves per slot).
 template <int K, typename V, typename T>
@@ -296,22 +327,23 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
                 b1[j] = tb[e1];
             }
             int k = 0;
+            // the next op's data is fetched while the current one runs
+            OpLite nxt = op_lite(op[rfirst], md[rfirst]);
             for (int o = rfirst; o < rfirst + rcount; ++o) {
-                const TileOp& po = op[o];
-                if (md[o].slot >= 0) {
+                const OpLite cur = nxt;
+                if (o + 1 < rfirst + rcount) nxt = op_lite(op[o + 1], md[o + 1]);
+                if (cur.slot >= 0) {
                     // share of Im <lambda| P |psi> right before the inverse
-                    const int pauli = md[o].pauli;
-                    const int gb = md[o].gbit;
                     double c = 0.0;
 #pragma unroll
                     for (int j = 0; j < PPT; ++j) {
                         double cj;
-                        if (pauli == LQ_PAULI_Z) {
+                        if (cur.pauli == LQ_PAULI_Z) {
                             const double v0 = (double)b0[j].x * a0[j].y - (double)b0[j].y * a0[j].x;
                             const double v1 = (double)b1[j].x * a1[j].y - (double)b1[j].y * a1[j].x;
                             const uint64_t g1 = g0[j] | gmask;
-                            cj = (((g0[j] >> gb) & 1ull) ? -v0 : v0) + (((g1 >> gb) & 1ull) ? -v1 : v1);
-                        } else if (pauli == LQ_PAULI_X) {
+                            cj = (((g0[j] >> cur.gbit) & 1ull) ? -v0 : v0) + (((g1 >> cur.gbit) & 1ull) ? -v1 : v1);
+                        } else if (cur.pauli == LQ_PAULI_X) {
                             // (X psi)_e0 = psi_e1, (X psi)_e1 = psi_e0
                             cj = ((double)b0[j].x * a1[j].y - (double)b0[j].y * a1[j].x) +
                                  ((double)b1[j].x * a0[j].y - (double)b1[j].y * a0[j].x);
@@ -328,12 +360,11 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
                     acc3 += k == 3 ? c : 0.0;
                     ++k;
                 }
-                const int kind = po.kind;
-                const uint64_t ctrl = po.ctrl;
-                if (kind == kTDiag) {
-                    const int s0 = po.sel0, s1 = po.sel1;
-                    const T r0 = (T)po.m[0], i0 = (T)po.m[1], r1 = (T)po.m[2], i1 = (T)po.m[3];
-                    const T r2 = (T)po.m[4], i2 = (T)po.m[5], r3 = (T)po.m[6], i3 = (T)po.m[7];
+                const uint64_t ctrl = cur.ctrl;
+                const T m0 = (T)cur.m[0], m1 = (T)cur.m[1], m2 = (T)cur.m[2], m3 = (T)cur.m[3];
+                const T m4 = (T)cur.m[4], m5 = (T)cur.m[5], m6 = (T)cur.m[6], m7 = (T)cur.m[7];
+                if (cur.kind == kTDiag) {
+                    const int s0 = cur.sel0, s1 = cur.sel1;
 #pragma unroll
                     for (int j = 0; j < PPT; ++j) {
 #pragma unroll
@@ -342,8 +373,8 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
                             const bool on = (g & ctrl) == ctrl;
                             const bool c0 = s0 >= 0 && ((g >> s0) & 1ull);
                             const bool c1 = s1 >= 0 && ((g >> s1) & 1ull);
-                            const T pr = on ? (c1 ? (c0 ? r3 : r2) : (c0 ? r1 : r0)) : (T)1;
-                            const T pi = on ? (c1 ? (c0 ? i3 : i2) : (c0 ? i1 : i0)) : (T)0;
+                            const T pr = on ? (c1 ? (c0 ? m6 : m4) : (c0 ? m2 : m0)) : (T)1;
+                            const T pi = on ? (c1 ? (c0 ? m7 : m5) : (c0 ? m3 : m1)) : (T)0;
                             if (h) {
                                 a1[j] = cmul(pr, pi, a1[j]);
                                 b1[j] = cmul(pr, pi, b1[j]);
@@ -353,48 +384,59 @@ __global__ void __launch_bounds__(kAThreads) k_tile_adj(V* __restrict__ psi, V* 
                             }
                         }
                     }
-                } else {
-                    // 2x2 on bit lb (controls never include lb)
-                    const T m0 = (T)po.m[0], m1 = (T)po.m[1], m2 = (T)po.m[2], m3 = (T)po.m[3];
-                    const T m4 = (T)po.m[4], m5 = (T)po.m[5], m6 = (T)po.m[6], m7 = (T)po.m[7];
+                } else if (cur.kind == kTPairMat) {
+                    // 2x2 on bit lb (controls never include lb); lanes whose
+                    // controls are not all 1 keep their values
 #pragma unroll
                     for (int j = 0; j < PPT; ++j) {
-                        if (ctrl && (g0[j] & ctrl) != ctrl) continue;
-                        V x0, x1, y0, y1;
-                        if (kind == kTPairMat) {
-                            x0 = cadd(cmul(m0, m1, a0[j]), cmul(m2, m3, a1[j]));
-                            x1 = cadd(cmul(m4, m5, a0[j]), cmul(m6, m7, a1[j]));
-                            y0 = cadd(cmul(m0, m1, b0[j]), cmul(m2, m3, b1[j]));
-                            y1 = cadd(cmul(m4, m5, b0[j]), cmul(m6, m7, b1[j]));
-                        } else if (kind == kTPairSwap) {
-                            x0 = a1[j];
-                            x1 = a0[j];
-                            y0 = b1[j];
-                            y1 = b0[j];
-                        } else if (kind == kTPairNegY) {
-                            x0 = cmul((T)0, (T)1, a1[j]);
-                            x1 = cmul((T)0, (T)-1, a0[j]);
-                            y0 = cmul((T)0, (T)1, b1[j]);
-                            y1 = cmul((T)0, (T)-1, b0[j]);
-                        } else {  // kTPairH: m0 = 1/sqrt(2)
-                            V sa, da, sb, db;
-                            sa.x = a0[j].x + a1[j].x;
-                            sa.y = a0[j].y + a1[j].y;
-                            da.x = a0[j].x - a1[j].x;
-                            da.y = a0[j].y - a1[j].y;
-                            sb.x = b0[j].x + b1[j].x;
-                            sb.y = b0[j].y + b1[j].y;
-                            db.x = b0[j].x - b1[j].x;
-                            db.y = b0[j].y - b1[j].y;
-                            x0 = cmul(m0, (T)0, sa);
-                            x1 = cmul(m0, (T)0, da);
-                            y0 = cmul(m0, (T)0, sb);
-                            y1 = cmul(m0, (T)0, db);
-                        }
-                        a0[j] = x0;
-                        a1[j] = x1;
-                        b0[j] = y0;
-                        b1[j] = y1;
+                        const bool on = (g0[j] & ctrl) == ctrl;
+                        const V x0 = cadd(cmul(m0, m1, a0[j]), cmul(m2, m3, a1[j]));
+                        const V x1 = cadd(cmul(m4, m5, a0[j]), cmul(m6, m7, a1[j]));
+                        const V y0 = cadd(cmul(m0, m1, b0[j]), cmul(m2, m3, b1[j]));
+                        const V y1 = cadd(cmul(m4, m5, b0[j]), cmul(m6, m7, b1[j]));
+                        a0[j] = vsel(on, x0, a0[j]);
+                        a1[j] = vsel(on, x1, a1[j]);
+                        b0[j] = vsel(on, y0, b0[j]);
+                        b1[j] = vsel(on, y1, b1[j]);
+                    }
+                } else if (cur.kind == kTPairSwap) {
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        const bool on = (g0[j] & ctrl) == ctrl;
+                        const V x0 = a0[j], y0 = b0[j];
+                        a0[j] = vsel(on, a1[j], a0[j]);
+                        a1[j] = vsel(on, x0, a1[j]);
+                        b0[j] = vsel(on, b1[j], b0[j]);
+                        b1[j] = vsel(on, y0, b1[j]);
+                    }
+                } else if (cur.kind == kTPairNegY) {
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        const bool on = (g0[j] & ctrl) == ctrl;
+                        const V x0 = cmul((T)0, (T)1, a1[j]), x1 = cmul((T)0, (T)-1, a0[j]);
+                        const V y0 = cmul((T)0, (T)1, b1[j]), y1 = cmul((T)0, (T)-1, b0[j]);
+                        a0[j] = vsel(on, x0, a0[j]);
+                        a1[j] = vsel(on, x1, a1[j]);
+                        b0[j] = vsel(on, y0, b0[j]);
+                        b1[j] = vsel(on, y1, b1[j]);
+                    }
+                } else {  // kTPairH: m0 = 1/sqrt(2)
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        const bool on = (g0[j] & ctrl) == ctrl;
+                        V sa, da, sb, db;
+                        sa.x = a0[j].x + a1[j].x;
+                        sa.y = a0[j].y + a1[j].y;
+                        da.x = a0[j].x - a1[j].x;
+                        da.y = a0[j].y - a1[j].y;
+                        sb.x = b0[j].x + b1[j].x;
+                        sb.y = b0[j].y + b1[j].y;
+                        db.x = b0[j].x - b1[j].x;
+                        db.y = b0[j].y - b1[j].y;
+                        a0[j] = vsel(on, cmul(m0, (T)0, sa), a0[j]);
+                        a1[j] = vsel(on, cmul(m0, (T)0, da), a1[j]);
+                        b0[j] = vsel(on, cmul(m0, (T)0, sb), b0[j]);
+                        b1[j] = vsel(on, cmul(m0, (T)0, db), b1[j]);
                     }
                 }
             }
@@ -723,9 +765,12 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
     const size_t nof = plf.ops.size(), nob = plb.ops.size();
     // instances per chunk: device states and partials within half the free
     // memory (at most 64 GiB), host op tables within 512 MiB
-    size_t freeb = 0, totb = 0;
-    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
-    const size_t budget = std::min<size_t>(freeb / 2, (size_t)64 << 30);
+    static size_t budget = 0;  // queried once: the device query is not free
+    if (!budget) {
+        size_t freeb = 0, totb = 0;
+        if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
+        budget = std::min<size_t>(freeb / 2, (size_t)64 << 30);
+    }
     const size_t per_dev = 2 * dim * amp + (size_t)std::max(nslots, 1) * nwp * 8 + 4096;
     const size_t per_host = (nof + nob) * sizeof(TileOp) + (size_t)std::max(nt, 1) * sizeof(PauliTermDev) + 64;
     int chunk = (int)std::min<size_t>((size_t)count, std::max<size_t>(1, budget / per_dev));

@@ -16,8 +16,6 @@
 
 namespace lq {
 
-int upload_plan(const std::vector<TileOp>& ops, const std::vector<TileGroup>& groups, TileOp** d_ops,
-                TileGroup** d_groups);
 void bind_template(const lq_gate* tmpl, int num_gates, const double* params, int num_params,
                    std::vector<lq_gate>& out);
 
@@ -185,30 +183,60 @@ static int blocks_for(uint64_t work) {
     return (int)b;
 }
 
-// Expectations of ninst states (dim amplitudes each, contiguous from psi).
-// obs_of[i] indexes `sums` (instances may share one sum).
-static int batch_expect(const void* psi, int dtype, uint64_t dim, int ninst, const std::vector<PauliSum>& sums,
-                        const std::vector<int>& obs_of, double* energies) {
-    // terms
+// One batch: ninst states of dim amplitudes run one pass plan (per-instance
+// op tables) from |0...0>, then every instance's Pauli-sum expectation.  The
+// host side prepares everything into pinned buffers first; the device work
+// is then one launch sequence on the library stream (fill, op upload, tile
+// passes, item upload, expectation, sums, result copy), which can be
+// captured once into a hipGraph and replayed for later batches of the same
+// shape: only buffer contents change between replays.
+static DevBuf g_bplan;
+static HostBuf g_hplan;
+
+struct BatchJob {
+    int n = 0, dtype = 0, ninst = 0;
+    uint64_t dim = 0;
+    const Plan* plan = nullptr;
+    size_t nops = 0, goff = 0, plan_bytes = 0;
+    int nitems = 0, blocks = 0;
+    size_t roff = 0, items_bytes = 0, terms_bytes = 0;
+};
+
+// Pinned space for the op tables of ninst instances (filled by the caller),
+// followed by the plan's groups.
+static int job_ops(BatchJob& j, TileOp** out) {
+    const size_t ob = j.nops * (size_t)j.ninst * sizeof(TileOp);
+    j.goff = (ob + 255) & ~(size_t)255;
+    j.plan_bytes = j.goff + j.plan->groups.size() * sizeof(TileGroup);
+    int rc = g_hplan.get(j.plan_bytes + 256);
+    if (!rc) rc = g_bplan.get(j.plan_bytes + 256);
+    if (rc) return rc;
+    if (!j.plan->groups.empty())
+        std::memcpy((char*)g_hplan.p + j.goff, j.plan->groups.data(), j.plan->groups.size() * sizeof(TileGroup));
+    *out = (TileOp*)g_hplan.p;
+    return LQ_OK;
+}
+
+// Expectation work items of the batch (obs_of[i] indexes `sums`).
+static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std::vector<int>& obs_of) {
     std::vector<int> term_base(sums.size());
     size_t nterms = 0;
     for (size_t k = 0; k < sums.size(); ++k) {
         term_base[k] = (int)nterms;
         nterms += sums[k].terms.size();
     }
-    // items
     std::vector<ExpItem> items;
-    std::vector<int> first(ninst), count(ninst);
+    std::vector<int> first(j.ninst), count(j.ninst);
     int blocks = 0;
-    for (int i = 0; i < ninst; ++i) {
+    for (int i = 0; i < j.ninst; ++i) {
         const PauliSum& ps = sums[obs_of[i]];
         first[i] = blocks;
         for (const PauliGroup& g : ps.groups) {
             ExpItem it;
             std::memset(&it, 0, sizeof(it));
-            it.state_off = (uint64_t)i * dim;
+            it.state_off = (uint64_t)i * j.dim;
             it.x = g.x;
-            it.work = g.x ? dim / 2 : dim;
+            it.work = g.x ? j.dim / 2 : j.dim;
             it.h = g.x ? 63 - __builtin_clzll(g.x) : -1;
             it.term_first = term_base[obs_of[i]] + g.first;
             it.term_count = g.count;
@@ -220,89 +248,182 @@ static int batch_expect(const void* psi, int dtype, uint64_t dim, int ninst, con
         }
         count[i] = blocks - first[i];
     }
-    if (blocks == 0) {
-        for (int i = 0; i < ninst; ++i) energies[i] = 0.0;
-        return LQ_OK;
-    }
-    size_t ib = items.size() * sizeof(ExpItem);
-    size_t tb = std::max<size_t>(nterms, 1) * sizeof(PauliTermDev);
-    size_t rb = (size_t)ninst * 2 * sizeof(int);
-    int rc = g_items.get(ib + rb + 256);
-    if (!rc) rc = g_terms.get(tb);
-    if (!rc) rc = g_part.get((size_t)blocks * sizeof(double));
-    if (!rc) rc = g_out.get((size_t)ninst * sizeof(double));
-    if (!rc) rc = g_hitems.get(ib + rb + 256);
-    if (!rc) rc = g_hterms.get(tb);
-    if (!rc) rc = g_hout.get((size_t)ninst * sizeof(double));
+    j.nitems = (int)items.size();
+    j.blocks = blocks;
+    const size_t ib = items.size() * sizeof(ExpItem);
+    const size_t rb = (size_t)j.ninst * 2 * sizeof(int);
+    j.roff = (ib + 255) & ~(size_t)255;
+    j.items_bytes = j.roff + rb;
+    j.terms_bytes = std::max<size_t>(nterms, 1) * sizeof(PauliTermDev);
+    int rc = g_items.get(j.items_bytes + 256);
+    if (!rc) rc = g_terms.get(j.terms_bytes);
+    if (!rc) rc = g_part.get((size_t)std::max(blocks, 1) * sizeof(double));
+    if (!rc) rc = g_out.get((size_t)j.ninst * sizeof(double));
+    if (!rc) rc = g_hitems.get(j.items_bytes + 256);
+    if (!rc) rc = g_hterms.get(j.terms_bytes);
+    if (!rc) rc = g_hout.get((size_t)j.ninst * sizeof(double));
     if (rc) return rc;
-    size_t roff = (ib + 255) & ~(size_t)255;
     char* hi = (char*)g_hitems.p;
-    std::memcpy(hi, items.data(), ib);
-    std::memcpy(hi + roff, first.data(), ninst * sizeof(int));
-    std::memcpy(hi + roff + ninst * sizeof(int), count.data(), ninst * sizeof(int));
+    if (ib) std::memcpy(hi, items.data(), ib);
+    std::memcpy(hi + j.roff, first.data(), j.ninst * sizeof(int));
+    std::memcpy(hi + j.roff + j.ninst * sizeof(int), count.data(), j.ninst * sizeof(int));
     PauliTermDev* ht = (PauliTermDev*)g_hterms.p;
     for (size_t k = 0; k < sums.size(); ++k)
         if (!sums[k].terms.empty())
             std::memcpy(ht + term_base[k], sums[k].terms.data(), sums[k].terms.size() * sizeof(PauliTermDev));
-    LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, hi, roff + rb, hipMemcpyHostToDevice, stream()));
-    LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, ht, nterms * sizeof(PauliTermDev), hipMemcpyHostToDevice, stream()));
-    const int* d_first = (const int*)((char*)g_items.p + roff);
-    const int* d_count = d_first + ninst;
-    if (dtype == LQ_C128)
-        hipLaunchKernelGGL(k_expect_items<double2>, dim3(blocks), dim3(kEThreads), 0, stream(), (const double2*)psi,
-                           (const ExpItem*)g_items.p, (int)items.size(), (const PauliTermDev*)g_terms.p,
-                           (double*)g_part.p);
-    else
-        hipLaunchKernelGGL(k_expect_items<float2>, dim3(blocks), dim3(kEThreads), 0, stream(), (const float2*)psi,
-                           (const ExpItem*)g_items.p, (int)items.size(), (const PauliTermDev*)g_terms.p,
-                           (double*)g_part.p);
-    LQ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sum_ranges, dim3(ninst), dim3(256), 0, stream(), (const double*)g_part.p, d_first, d_count,
-                       (double*)g_out.p);
-    LQ_LAUNCH_CHECK();
-    LQ_HIP_CHECK(hipMemcpyAsync(g_hout.p, g_out.p, ninst * sizeof(double), hipMemcpyDeviceToHost, stream()));
-    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-    std::memcpy(energies, g_hout.p, ninst * sizeof(double));
     return LQ_OK;
 }
 
-// Run `ninst` circuits that share one structure (`plan`, built from
-// instance 0) from |0...0>, instance i with op table inst_ops[i].
-static int batch_run(int n, int dtype, const Plan& plan, const std::vector<TileOp>& inst_ops, int ninst,
-                     void* psi) {
-    const uint64_t dim = 1ull << n;
-    const size_t amp = dtype == LQ_C128 ? 16 : 8;
-    (void)amp;
-    uint64_t total = dim * (uint64_t)ninst;
-    unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
-    if (dtype == LQ_C128)
-        hipLaunchKernelGGL(k_fill_zero_states<double2>, dim3(g), dim3(256), 0, stream(), (double2*)psi, dim, total);
+// The device work of a prepared job (capturable when the plan has no
+// renormalisation step).
+static int job_launch(const BatchJob& j, void* psi) {
+    const uint64_t total = j.dim * (uint64_t)j.ninst;
+    const unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+    if (j.dtype == LQ_C128)
+        hipLaunchKernelGGL(k_fill_zero_states<double2>, dim3(g), dim3(256), 0, stream(), (double2*)psi, j.dim, total);
     else
-        hipLaunchKernelGGL(k_fill_zero_states<float2>, dim3(g), dim3(256), 0, stream(), (float2*)psi, dim, total);
+        hipLaunchKernelGGL(k_fill_zero_states<float2>, dim3(g), dim3(256), 0, stream(), (float2*)psi, j.dim, total);
     LQ_LAUNCH_CHECK();
-    TileOp* d_ops = nullptr;
-    TileGroup* d_groups = nullptr;
-    int rc = upload_plan(inst_ops, plan.groups, &d_ops, &d_groups);
-    if (rc) return rc;
-    const uint64_t ostride = plan.ops.size();
-    for (const Step& s : plan.steps) {
+    LQ_HIP_CHECK(hipMemcpyAsync(g_bplan.p, g_hplan.p, j.plan_bytes, hipMemcpyHostToDevice, stream()));
+    const TileOp* d_ops = (const TileOp*)g_bplan.p;
+    const TileGroup* d_groups = (const TileGroup*)((char*)g_bplan.p + j.goff);
+    for (const Step& s : j.plan->steps) {
+        int rc;
         if (s.normalize) {
             // MatrixGate renormalisation: per instance
-            for (int i = 0; i < ninst; ++i) {
+            for (int i = 0; i < j.ninst; ++i) {
                 lq_state tmp;
-                tmp.n = n;
-                tmp.dtype = dtype;
-                tmp.dim = dim;
-                tmp.amp_bytes = dtype == LQ_C128 ? 16 : 8;
-                tmp.d_amps = (char*)psi + (size_t)i * dim * tmp.amp_bytes;
+                tmp.n = j.n;
+                tmp.dtype = j.dtype;
+                tmp.dim = j.dim;
+                tmp.amp_bytes = j.dtype == LQ_C128 ? 16 : 8;
+                tmp.d_amps = (char*)psi + (size_t)i * j.dim * tmp.amp_bytes;
                 rc = normalize_device(&tmp, nullptr);
                 if (rc) return rc;
             }
             continue;
         }
-        rc = launch_tile_pass(psi, dtype, s.pass, d_ops, d_groups, (uint64_t)ninst, dim, ostride);
+        rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)j.ninst, j.dim, j.nops);
         if (rc) return rc;
     }
+    if (j.blocks > 0) {
+        LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, g_hitems.p, j.items_bytes, hipMemcpyHostToDevice, stream()));
+        LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, g_hterms.p, j.terms_bytes, hipMemcpyHostToDevice, stream()));
+        const int* d_first = (const int*)((char*)g_items.p + j.roff);
+        const int* d_count = d_first + j.ninst;
+        if (j.dtype == LQ_C128)
+            hipLaunchKernelGGL(k_expect_items<double2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
+                               (const double2*)psi, (const ExpItem*)g_items.p, j.nitems,
+                               (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+        else
+            hipLaunchKernelGGL(k_expect_items<float2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
+                               (const float2*)psi, (const ExpItem*)g_items.p, j.nitems,
+                               (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+        LQ_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sum_ranges, dim3(j.ninst), dim3(256), 0, stream(), (const double*)g_part.p, d_first,
+                           d_count, (double*)g_out.p);
+        LQ_LAUNCH_CHECK();
+        LQ_HIP_CHECK(hipMemcpyAsync(g_hout.p, g_out.p, j.ninst * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    }
+    return LQ_OK;
+}
+
+// hipGraphs of job launch sequences, keyed by everything the launches take
+// as arguments (sizes, pass geometry, buffer addresses).  A key seen for the
+// first time runs eagerly; the second time it is captured; later it replays.
+static bool g_graphs_on = true;
+static long g_graph_captures = 0, g_graph_replays = 0;
+struct GraphEntry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    long last_use = 0;
+};
+static std::vector<GraphEntry> g_graphs;
+static std::vector<std::vector<uint64_t>> g_seen_keys;
+static long g_graph_clock = 0;
+
+static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
+    std::vector<uint64_t> k = {(uint64_t)j.n, (uint64_t)j.dtype, (uint64_t)j.ninst, j.nops, j.goff, j.plan_bytes,
+                               (uint64_t)j.nitems, (uint64_t)j.blocks, j.roff, j.items_bytes, j.terms_bytes,
+                               (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p, (uint64_t)g_items.p,
+                               (uint64_t)g_hitems.p, (uint64_t)g_terms.p, (uint64_t)g_hterms.p, (uint64_t)g_part.p,
+                               (uint64_t)g_out.p, (uint64_t)g_hout.p, j.plan->steps.size()};
+    for (const Step& s : j.plan->steps) {
+        const TilePass& p = s.pass;
+        k.push_back(s.normalize ? 1 : 0);
+        k.push_back(((uint64_t)(uint32_t)p.L << 32) | (uint32_t)p.hb);
+        for (int h = 0; h < p.hb; ++h) k.push_back((uint64_t)p.hpos[h]);
+        k.push_back(((uint64_t)(uint32_t)p.op_first << 32) | (uint32_t)p.op_count);
+        k.push_back(((uint64_t)(uint32_t)p.group_first << 32) | (uint32_t)p.group_count);
+        k.push_back(((uint64_t)(uint32_t)p.rich << 32));
+        k.push_back(p.ntiles);
+    }
+    return k;
+}
+
+// Run a prepared job and wait for its energies (in g_hout).
+static int job_run(const BatchJob& j, void* psi, double* energies) {
+    bool graphable = g_graphs_on;
+    for (const Step& s : j.plan->steps)
+        if (s.normalize) graphable = false;
+    int rc = LQ_OK;
+    if (graphable) {
+        std::vector<uint64_t> key = job_key(j, psi);
+        GraphEntry* hit = nullptr;
+        for (GraphEntry& e : g_graphs)
+            if (e.key == key) hit = &e;
+        if (!hit && std::find(g_seen_keys.begin(), g_seen_keys.end(), key) != g_seen_keys.end()) {
+            // second sighting: capture the launch sequence
+            LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+            LQ_HIP_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            rc = job_launch(j, psi);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(stream(), &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error("graph capture failed: %s", hipGetErrorString(e));
+                return LQ_ERR_DEVICE;
+            }
+            GraphEntry ne;
+            ne.key = key;
+            e = hipGraphInstantiate(&ne.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                set_error("graph instantiation failed: %s", hipGetErrorString(e));
+                return LQ_ERR_DEVICE;
+            }
+            if (g_graphs.size() >= 16) {
+                auto old = std::min_element(g_graphs.begin(), g_graphs.end(), [](const GraphEntry& a,
+                                                                                 const GraphEntry& b) {
+                    return a.last_use < b.last_use;
+                });
+                (void)hipGraphExecDestroy(old->exec);
+                g_graphs.erase(old);
+            }
+            g_graphs.push_back(ne);
+            hit = &g_graphs.back();
+            ++g_graph_captures;
+        }
+        if (hit) {
+            hit->last_use = ++g_graph_clock;
+            LQ_HIP_CHECK(hipGraphLaunch(hit->exec, stream()));
+            ++g_graph_replays;
+        } else {
+            g_seen_keys.push_back(key);
+            if (g_seen_keys.size() > 64) g_seen_keys.erase(g_seen_keys.begin());
+            rc = job_launch(j, psi);
+        }
+    } else {
+        rc = job_launch(j, psi);
+    }
+    if (rc) return rc;
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+    if (j.blocks > 0) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
+    else
+        for (int i = 0; i < j.ninst; ++i) energies[i] = 0.0;
     return LQ_OK;
 }
 
@@ -358,17 +479,28 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
                 if (u.empty() || u.back() != g) u.push_back(g);
             }
     const size_t nops = plan.ops.size();
+    std::vector<int> op_of(num_gates, -1);  // each gate is one op of the plan
+    for (size_t oo = 0; oo < nops; ++oo) op_of[plan.op_src[oo]] = (int)oo;
     std::vector<double> energies(ninst);
     const int chunk = chunk_instances(dim, amp, ninst);
-    std::vector<TileOp> ops;
     std::vector<PauliSum> sums(1, ps);
     std::vector<double> p(params, params + num_params);
+    std::vector<Prim> one;
     for (int c0 = 0; c0 < ninst; c0 += chunk) {
         const int cn = std::min(chunk, ninst - c0);
-        ops.resize(nops * (size_t)cn);
+        BatchJob job;
+        job.n = n;
+        job.dtype = dtype;
+        job.ninst = cn;
+        job.dim = dim;
+        job.plan = &plan;
+        job.nops = nops;
+        TileOp* ops = nullptr;
+        rc = job_ops(job, &ops);
+        if (rc) return rc;
         for (int k = 0; k < cn; ++k) {
             const int inst = c0 + k;
-            TileOp* o = ops.data() + nops * (size_t)k;
+            TileOp* o = ops + nops * (size_t)k;
             std::memcpy(o, plan.ops.data(), nops * sizeof(TileOp));
             if (inst == 0) continue;
             // instance 2i+1: theta_i + s, instance 2i+2: theta_i - s
@@ -379,20 +511,17 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
                 lq_gate gg = tmpl[g];
                 for (int kk = 0; kk < 3; ++kk)
                     if (gg.param[kk] >= 0) gg.theta[kk] = p[gg.param[kk]];
-                std::vector<Prim> one;
+                one.clear();
                 rc = gates_to_prims(n, &gg, 1, one);
                 if (rc) return rc;
-                for (size_t oo = 0; oo < nops; ++oo)
-                    if (plan.op_src[oo] == g) std::memcpy(o[oo].m, one[0].m, sizeof(o[oo].m));
+                if (op_of[g] >= 0) std::memcpy(o[op_of[g]].m, one[0].m, sizeof(o[op_of[g]].m));
             }
             p[i] = params[i];
         }
-        rc = g_states.get((size_t)cn * dim * amp);
-        if (rc) return rc;
-        rc = batch_run(n, dtype, plan, ops, cn, g_states.p);
-        if (rc) return rc;
         std::vector<int> obs_of(cn, 0);
-        rc = batch_expect(g_states.p, dtype, dim, cn, sums, obs_of, energies.data() + c0);
+        rc = job_expect(job, sums, obs_of);
+        if (!rc) rc = g_states.get((size_t)cn * dim * amp);
+        if (!rc) rc = job_run(job, g_states.p, energies.data() + c0);
         if (rc) return rc;
     }
     (void)fp;
@@ -473,10 +602,18 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
         const size_t nops = plan.ops.size();
         const int total = (int)members.size();
         const int chunk = chunk_instances(dim, amp, total);
-        std::vector<TileOp> ops;
         for (int c0 = 0; c0 < total; c0 += chunk) {
             const int cn = std::min(chunk, total - c0);
-            ops.resize(nops * (size_t)cn);
+            BatchJob job;
+            job.n = n;
+            job.dtype = dtype;
+            job.ninst = cn;
+            job.dim = dim;
+            job.plan = &plan;
+            job.nops = nops;
+            TileOp* ops = nullptr;
+            rc = job_ops(job, &ops);
+            if (rc) return rc;
             std::vector<PauliSum> csums;
             std::vector<int> obs_of(cn);
             for (int k = 0; k < cn; ++k) {
@@ -484,19 +621,30 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
                 std::vector<Prim> pr;
                 rc = gates_to_prims(n, circuits[c], ng, pr);
                 if (rc) return rc;
-                instance_ops(plan, pr, fp, ops.data() + nops * (size_t)k);
+                instance_ops(plan, pr, fp, ops + nops * (size_t)k);
                 csums.push_back(sums[c]);
                 obs_of[k] = k;
             }
-            rc = g_states.get((size_t)cn * dim * amp);
-            if (rc) return rc;
-            rc = batch_run(n, dtype, plan, ops, cn, g_states.p);
-            if (rc) return rc;
             std::vector<double> e(cn);
-            rc = batch_expect(g_states.p, dtype, dim, cn, csums, obs_of, e.data());
+            rc = job_expect(job, csums, obs_of);
+            if (!rc) rc = g_states.get((size_t)cn * dim * amp);
+            if (!rc) rc = job_run(job, g_states.p, e.data());
             if (rc) return rc;
             for (int k = 0; k < cn; ++k) energies[members[c0 + k]] = e[k];
         }
     }
+    return LQ_OK;
+}
+
+extern "C" int lq_set_batch_graphs(int enable) {
+    clear_error();
+    g_graphs_on = enable != 0;
+    return LQ_OK;
+}
+
+extern "C" int lq_batch_graph_stats(long long* captures, long long* replays) {
+    clear_error();
+    if (captures) *captures = g_graph_captures;
+    if (replays) *replays = g_graph_replays;
     return LQ_OK;
 }

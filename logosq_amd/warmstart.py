@@ -287,7 +287,8 @@ def cmd_generate(a):
         state = (solved, X, E, it)
         print("resuming: %d of %d solved" % (solved.sum(), len(J)), flush=True)
     opt = Adam(w["learning_rate"], w["max_iterations"]).with_tolerance(w["tolerance"])
-    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, seeds=1, log=lambda m: print(m, flush=True),
+    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, seeds=1,
+                                     log=lambda m: print(m, flush=True),
                                      state=state, time_limit=a.time_limit,
                                      checkpoint=lambda s: _save(a.out, s, J, theta0, w))
     print("done: %d of %d solved" % (state[0].sum(), len(J)), flush=True)
@@ -375,7 +376,7 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     g = sub.add_parser("generate", help="converged VQE solutions of the training couplings")
     g.add_argument("--out", required=True)
-    g.add_argument("--batch", type=int, default=64)
+    g.add_argument("--batch", type=int, default=16)
     g.add_argument("--time-limit", type=float, default=None)
     e = sub.add_parser("evaluate", help="train on the GPU, then warm vs plain on held-out couplings")
     e.add_argument("--data", required=True)
