@@ -166,6 +166,20 @@ int lq_state_inner_product(const lq_state* a, const lq_state* b, double out[2]);
 /* probabilities (mod.rs:277-279): len = 2^n doubles. */
 int lq_state_probabilities(const lq_state* st, double* out, uint64_t len);
 
+/* ---- Measurement (src/states/mod.rs:180-262, src/circuits/mod.rs:215-247)
+ * The reference draws from thread_rng; here the caller supplies the uniforms
+ * or a seed, so results are reproducible. */
+/* p1[b] = probability that bit b of the basis index is 1, b = 0..n-1 (f64). */
+int lq_bit_probabilities(const lq_state* st, double* p1);
+/* State::measure_qubit (mod.rs:196-231): mask = 1 << qubit_idx (bit
+ * qubit_idx of the index, as the reference does), outcome 1 when u < P(1),
+ * then the projection and State::normalize.  u is a uniform in [0, 1). */
+int lq_measure_qubit(lq_state* st, int qubit_idx, double u, int* result);
+/* State::measure repeated `shots` times (mod.rs:180-185, 241-255): indices
+ * drawn from |a_i|^2 (WeightedIndex rule: first i whose cumulative weight
+ * exceeds u * total) with uniforms from SplitMix64(seed); no collapse. */
+int lq_sample_shots(const lq_state* st, uint64_t seed, int64_t shots, uint64_t* out);
+
 /* ---- Gates (Gate::apply, src/gates/mod.rs:26-35) ------------------------ */
 /* Apply one gate with the kernel for its kind (the per-gate drop-in). */
 int lq_apply_gate(lq_state* st, const lq_gate* gate);

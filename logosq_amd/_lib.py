@@ -48,6 +48,9 @@ SIGNATURES = {
     "lq_state_norm_sqr": [c_void_p, P_double],
     "lq_state_inner_product": [c_void_p, c_void_p, P_double],
     "lq_state_probabilities": [c_void_p, P_double, c_u64],
+    "lq_bit_probabilities": [c_void_p, P_double],
+    "lq_measure_qubit": [c_void_p, c_int, c_double, P_int],
+    "lq_sample_shots": [c_void_p, c_u64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)],
     "lq_apply_gate": [c_void_p, P_gate],
     "lq_apply_1q": [c_void_p, c_int, P_double],
     "lq_apply_ctrl_1q": [c_void_p, P_int, c_int, c_int, P_double],

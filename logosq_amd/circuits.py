@@ -55,6 +55,49 @@ class Operation:
         return "%s(qubits: %s)" % (self._name, self._qubits)
 
 
+def _local_matrix(gate, qubits):
+    """The gate's matrix on its operation's k qubits (sub-index: bit of
+    qubits[0] first), from its action on a k-qubit device state whose qubits
+    keep the order of the originals (the reference's quirks depend on it)."""
+    from .states import State
+    k = len(qubits)
+    local = {q: r for r, q in enumerate(sorted(qubits))}
+    g = abi.LqGate()
+    ctypes.memmove(ctypes.addressof(g), ctypes.addressof(gate), ctypes.sizeof(gate))
+    for i in range(3):
+        if g.qubits[i] >= 0:
+            g.qubits[i] = local[g.qubits[i]]
+
+    def basis(sub):
+        return sum(((sub >> (k - 1 - i)) & 1) << (k - 1 - local[qubits[i]]) for i in range(k))
+
+    st = State(k)
+    m = np.zeros((1 << k, 1 << k), dtype=np.complex128)
+    for col in range(1 << k):
+        call("lq_state_set_basis", st.handle, basis(col))
+        call("lq_apply_gate", st.handle, ctypes.byref(g))
+        v = st.vector()
+        for row in range(1 << k):
+            m[row, col] = v[basis(row)]
+    return m
+
+
+def _full_matrix(op, n):
+    """An operation's full 2^n matrix probed on basis states (mod.rs:456-467)."""
+    from .states import State
+    dim = 1 << n
+    m = np.zeros((dim, dim), dtype=np.complex128)
+    st = State(n)
+    for col in range(dim):
+        call("lq_state_set_basis", st.handle, col)
+        if isinstance(op.gate, abi.LqGate):
+            call("lq_apply_gate", st.handle, ctypes.byref(op.gate))
+        else:
+            op.gate.apply(st)
+        m[:, col] = st.vector()
+    return m
+
+
 class Circuit:
     def __init__(self, num_qubits):
         self._n = int(num_qubits)
@@ -272,6 +315,55 @@ class Circuit:
         self._flush(state, run, mode)
 
     execute_without_noise = execute
+
+    def execute_and_measure(self, uniforms=None):
+        """Circuit::execute_and_measure (mod.rs:215-233, the default parallel
+        build): run from |0...0>, then measure_qubit(i) for i = 0..n-1, each on
+        its own copy of the final state, i.e. bit i against P(bit i = 1).
+        `uniforms` (n values in [0, 1)) replace the reference's thread_rng."""
+        from .states import State
+        st = State.zero_state(self._n)
+        self.execute(st)
+        p1 = st.bit_probabilities()
+        u = np.random.random(self._n) if uniforms is None else np.asarray(uniforms, dtype=np.float64)
+        return [1 if u[i] < p1[i] else 0 for i in range(self._n)]
+
+    def execute_on_backend(self, backend):
+        """Circuit::execute_on_backend (mod.rs:444-504) with descriptor
+        dispatch.  The reference probes every operation's full 2^n matrix on
+        basis states (mod.rs:456-467) and reads from it the block on the
+        operation's qubits with every other qubit 0 (mod.rs:508-590); here
+        that block is read from the gate itself, applied to a state of as many
+        qubits as the operation has.  Three-or-more-qubit operations go to
+        apply_three_qubit_matrix / apply_full_matrix with the full matrix, as
+        in the reference; on the device State that full-state operator is
+        applied as the gate followed by State::normalize."""
+        from .states import State
+        if backend.num_qubits() != self._n:
+            raise CircuitQubitMismatch(abi.LQ_ERR_CIRCUIT_QUBIT_MISMATCH,
+                                       "Circuit qubit count mismatch: circuit has %d, state has %d"
+                                       % (self._n, backend.num_qubits()))
+        for op in self._ops:
+            qs = op.qubits()
+            if len(qs) in (1, 2) and isinstance(op.gate, abi.LqGate):
+                m = _local_matrix(op.gate, qs)
+                if len(qs) == 1:
+                    backend.apply_single_qubit_matrix(qs[0], m)
+                else:
+                    backend.apply_two_qubit_matrix(qs[0], qs[1], m)
+                continue
+            if isinstance(backend, State):
+                if isinstance(op.gate, abi.LqGate):
+                    call("lq_apply_gate", backend.handle, ctypes.byref(op.gate))
+                else:
+                    op.gate.apply(backend)
+                backend.normalize()
+                continue
+            full = _full_matrix(op, self._n)
+            if len(qs) == 3:
+                backend.apply_three_qubit_matrix(qs[0], qs[1], qs[2], full)
+            else:
+                backend.apply_full_matrix(full)
 
     @staticmethod
     def _flush(state, run, mode):

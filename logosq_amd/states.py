@@ -9,7 +9,7 @@ import math
 import numpy as np
 
 from . import abi
-from ._lib import call, InvalidStateDimension
+from ._lib import call, DimensionMismatch, InvalidQubitIndex, InvalidStateDimension
 
 C128 = abi.LQ_C128
 C64 = abi.LQ_C64
@@ -20,7 +20,32 @@ def _as_interleaved(vec):
     return v, v.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-class State:
+class QuantumStateBackend:
+    """trait QuantumStateBackend (src/states/backend.rs:14-60): what
+    Circuit.execute_on_backend drives."""
+
+    def num_qubits(self):
+        raise NotImplementedError
+
+    def apply_single_qubit_matrix(self, qubit, matrix):
+        raise NotImplementedError
+
+    def apply_two_qubit_matrix(self, control, target, matrix):
+        raise NotImplementedError
+
+    def apply_three_qubit_matrix(self, q1, q2, q3, matrix):
+        raise NotImplementedError
+
+    def apply_full_matrix(self, matrix):
+        raise NotImplementedError
+
+
+def _mat_ptr(m):
+    m = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+    return m, m.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class State(QuantumStateBackend):
     """A 2^n complex state vector on the GPU (complex128 or complex64)."""
 
     def __init__(self, num_qubits, dtype=C128):
@@ -119,6 +144,45 @@ class State:
             return 0.0
         return float(abs(self.vector()[basis_state]) ** 2)
 
+    # ---- measurement (mod.rs:180-262); the reference's RNG is unseeded, so
+    # these take uniforms or a seed ----
+    def bit_probabilities(self):
+        """p1[b] = P(bit b of the basis index is 1)."""
+        out = np.empty(self._n, dtype=np.float64)
+        call("lq_bit_probabilities", self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def measure_qubit(self, qubit_idx, u=None):
+        """State::measure_qubit (mod.rs:196-231): mask 1 << qubit_idx, outcome
+        1 when u < P(1), then project and normalise."""
+        if u is None:
+            u = float(np.random.random())
+        r = ctypes.c_int()
+        call("lq_measure_qubit", self._h, int(qubit_idx), float(u), ctypes.byref(r))
+        return r.value
+
+    def sample(self, shots, seed=0):
+        """`shots` draws of State::measure (no collapse) as basis indices."""
+        out = np.empty(int(shots), dtype=np.uint64)
+        call("lq_sample_shots", self._h, int(seed) & ((1 << 64) - 1), int(shots),
+             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        return out
+
+    def measure(self, seed=None):
+        """State::measure (mod.rs:180-185): one basis index drawn from |a|^2."""
+        if seed is None:
+            seed = int(np.random.randint(0, 2 ** 63))
+        return int(self.sample(1, seed)[0])
+
+    def measure_shots(self, n_shots, seed=None):
+        """measure_shots_parallel / measure_shots (mod.rs:241-262): counts."""
+        if seed is None:
+            seed = int(np.random.randint(0, 2 ** 63))
+        idx, cnt = np.unique(self.sample(n_shots, seed), return_counts=True)
+        return {int(i): int(c) for i, c in zip(idx, cnt)}
+
+    measure_shots_parallel = measure_shots
+
     def inner_product(self, other):
         out = (ctypes.c_double * 2)()
         call("lq_state_inner_product", self._h, other._h, out)
@@ -133,6 +197,39 @@ class State:
 
     def apply_gate(self, gate):
         gate.apply(self)
+
+    # ---- QuantumStateBackend for the dense state (src/states/mod.rs:394-487):
+    # each method is the reference's expand-then-apply_full_matrix, i.e. the
+    # block on its qubits followed by State::normalize, without building the
+    # 2^n x 2^n matrix ----
+    def apply_single_qubit_matrix(self, qubit, matrix):
+        if qubit < 0 or qubit >= self._n:
+            raise InvalidQubitIndex(abi.LQ_ERR_INVALID_QUBIT_INDEX,
+                                    "Invalid qubit index: %d (must be < %d)" % (qubit, self._n))
+        m, p = _mat_ptr(np.asarray(matrix).reshape(2, 2))
+        call("lq_apply_1q", self._h, int(qubit), p)
+        self.normalize()
+
+    def apply_two_qubit_matrix(self, control, target, matrix):
+        if max(control, target) >= self._n or min(control, target) < 0:
+            raise InvalidQubitIndex(abi.LQ_ERR_INVALID_QUBIT_INDEX,
+                                    "Invalid qubit index: %d (must be < %d)" % (max(control, target), self._n))
+        m, p = _mat_ptr(np.asarray(matrix).reshape(4, 4))
+        call("lq_apply_2q", self._h, int(control), int(target), p)
+        self.normalize()
+
+    def apply_three_qubit_matrix(self, q1, q2, q3, matrix):
+        # the reference applies `matrix` as the full-state operator (mod.rs:459-469)
+        self.apply_full_matrix(matrix)
+
+    def apply_full_matrix(self, matrix):
+        m = np.asarray(matrix)
+        dim = 1 << self._n
+        if m.ndim != 2 or m.shape[1] != dim or m.shape[0] != dim:
+            raise DimensionMismatch(abi.LQ_ERR_DIMENSION_MISMATCH, "Dimension mismatch: expected %d, got %d"
+                                    % (dim, m.shape[1] if m.ndim == 2 else -1))
+        m, p = _mat_ptr(m)
+        call("lq_apply_dense", self._h, p, dim)
 
     def __del__(self):
         h = getattr(self, "_h", None)
