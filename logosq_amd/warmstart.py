@@ -118,7 +118,7 @@ def hamiltonians(num_qubits, J):
     return [heisenberg_ring(num_qubits, *map(float, j)) for j in np.atleast_2d(J)]
 
 
-def solve(ansatz, J, starts, optimizer, dtype=None, batch=64):
+def solve(ansatz, J, starts, optimizer, dtype=None, batch=64, log=None):
     """VQE::run (Adam, adjoint gradient) for each coupling J[i] from
     starts[i], in lock-step batches on the GPU.  Returns (params, energies,
     histories, converged)."""
@@ -137,18 +137,22 @@ def solve(ansatz, J, starts, optimizer, dtype=None, batch=64):
             es.append(r.ground_state_energy)
             hs.append(np.array(r.convergence_history))
         cs += list(conv)
+        if log:
+            log("solved %d of %d" % (len(xs), len(J)))
     return np.array(xs), np.array(es), hs, np.array(cs)
 
 
 def generate_by_continuation(ansatz, J, theta0, optimizer, batch=64, seeds=None, log=None, state=None,
-                             time_limit=None, checkpoint=None):
+                             time_limit=None, checkpoint=None, optimizer_continue=None):
     """Converged VQE solutions for every coupling in J, grown outward from a
     few seeds: the seeds start from theta0; after that each round takes the
     `batch` unsolved couplings closest to the solved set and starts each from
     the solution of its nearest solved neighbour, so the solutions vary
     smoothly with J.  `state` (solved, X, E, iters) resumes an earlier run;
     `time_limit` (s) stops after the round that crosses it; `checkpoint` is
-    called with the state after every round.  Returns the state."""
+    called with the state after every round; `optimizer_continue` (default
+    `optimizer`) runs the rounds that start from a solved neighbour.
+    Returns the state."""
     import time
     t_start = time.perf_counter()
     J = np.asarray(J, dtype=np.float64)
@@ -180,7 +184,8 @@ def generate_by_continuation(ansatz, J, theta0, optimizer, batch=64, seeds=None,
             take = rest[np.argsort(near, kind="stable")[:batch]]
             nb = S[np.argmin(np.linalg.norm(J[take][:, None, :] - J[S][None, :, :], axis=2), axis=1)]
             st = X[nb]
-        x, e, h, _ = solve(ansatz, J[take], st, optimizer, batch=batch)
+        opt = optimizer if not solved.any() or optimizer_continue is None else optimizer_continue
+        x, e, h, _ = solve(ansatz, J[take], st, opt, batch=batch)
         X[take], E[take] = x, e
         iters[take] = [len(t) for t in h]
         solved[take] = True
@@ -255,7 +260,10 @@ def choose_starts(ws, ansatz, J, dtype=None):
 
 # ---- command line: data generation and evaluation on the GPU --------------------------
 WORKLOAD = {"num_qubits": 20, "layers": 8, "train": 4096, "held_out": 256, "seed": 5,
-            "learning_rate": 0.05, "max_iterations": 600, "tolerance": 1e-8}
+            "learning_rate": 0.05, "max_iterations": 600, "tolerance": 1e-8,
+            # runs that start from a solved neighbour's parameters; the first
+            # 993 training couplings were generated with 600 here
+            "continue_max_iterations": 250}
 
 
 def _save(path, state, J, theta0, meta):
@@ -287,7 +295,8 @@ def cmd_generate(a):
         state = (solved, X, E, it)
         print("resuming: %d of %d solved" % (solved.sum(), len(J)), flush=True)
     opt = Adam(w["learning_rate"], w["max_iterations"]).with_tolerance(w["tolerance"])
-    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, seeds=1,
+    opt_c = Adam(w["learning_rate"], w["continue_max_iterations"]).with_tolerance(w["tolerance"])
+    state = generate_by_continuation(ans, J, theta0, opt, batch=a.batch, seeds=1, optimizer_continue=opt_c,
                                      log=lambda m: print(m, flush=True),
                                      state=state, time_limit=a.time_limit,
                                      checkpoint=lambda s: _save(a.out, s, J, theta0, w))
@@ -317,7 +326,7 @@ def pick_learning_rate(ans, J, starts, grid, max_iterations, tolerance, dtype=No
     return min(grid, key=lambda lr: (med[lr], lr)), med
 
 
-def evaluate(ans, ws, J, theta0, lr_plain, lr_warm, max_iterations, tolerance, batch=64, dtype=None):
+def evaluate(ans, ws, J, theta0, lr_plain, lr_warm, max_iterations, tolerance, batch=64, dtype=None, log=None):
     """Plain and warm-started VQE::run (Adam, adjoint gradient: the same
     trajectory as parameter shift) on couplings J.  Per coupling: iterations
     until the energy is within 1e-4 of the lower of the two converged
@@ -325,9 +334,10 @@ def evaluate(ans, ws, J, theta0, lr_plain, lr_warm, max_iterations, tolerance, b
     from .optimization import Adam
     starts, used, ep, e0 = choose_starts(ws, ans, J, dtype)
     _, e_plain, h_plain, _ = solve(ans, J, np.tile(theta0, (len(J), 1)),
-                                   Adam(lr_plain, max_iterations).with_tolerance(tolerance), dtype, batch)
+                                   Adam(lr_plain, max_iterations).with_tolerance(tolerance), dtype, batch,
+                                   log and (lambda m: log("plain: " + m)))
     _, e_warm, h_warm, _ = solve(ans, J, starts, Adam(lr_warm, max_iterations).with_tolerance(tolerance), dtype,
-                                 batch)
+                                 batch, log and (lambda m: log("warm: " + m)))
     rows = []
     for i in range(len(J)):
         e_star = min(e_plain[i], e_warm[i])
@@ -336,6 +346,9 @@ def evaluate(ans, ws, J, theta0, lr_plain, lr_warm, max_iterations, tolerance, b
                      "e_start_plain": float(e0[i]), "e_start_proposal": float(ep[i]), "used_proposal": bool(used[i]),
                      "iters_plain": iterations_to(h_plain[i], e_star + 1e-4),
                      "iters_warm": iterations_to(h_warm[i], e_star + 1e-4),
+                     # against the plain method's own converged energy
+                     "iters_plain_to_plain": iterations_to(h_plain[i], e_plain[i] + 1e-4),
+                     "iters_warm_to_plain": iterations_to(h_warm[i], e_plain[i] + 1e-4),
                      "len_plain": len(h_plain[i]), "len_warm": len(h_warm[i])})
     return rows
 
@@ -348,20 +361,26 @@ def cmd_evaluate(a):
     ans = ry_rz_cz_ring_ansatz(meta["num_qubits"], meta["layers"])
     ws, hist = WarmStart.train(J, X, theta0, epochs=a.epochs, learning_rate=a.lr)
     print("trained: loss %.4g -> %.4g" % (hist[0], hist[-1]), flush=True)
-    Jte = couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][:a.count]
-    # learning rate per method on validation couplings (neither training nor held out)
-    Jval = couplings(a.validation, meta["seed"] + 1)
-    grid = tuple(float(v) for v in a.lr_grid.split(","))
-    lr_p, med_p = pick_learning_rate(ans, Jval, np.tile(theta0, (len(Jval), 1)), grid, a.max_iterations,
-                                     a.tolerance)
-    lr_w, med_w = pick_learning_rate(ans, Jval, choose_starts(ws, ans, Jval)[0], grid, a.max_iterations,
-                                     a.tolerance)
-    print("learning rates: plain %g %s, warm %g %s" % (lr_p, med_p, lr_w, med_w), flush=True)
-    rows = evaluate(ans, ws, Jte, theta0, lr_p, lr_w, a.max_iterations, a.tolerance)
+    Jte = couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][a.offset:a.offset + a.count]
+    log = lambda m: print(m, flush=True)
+    if a.lr_plain and a.lr_warm:
+        lr_p, lr_w, med_p, med_w = a.lr_plain, a.lr_warm, None, None
+    else:
+        # learning rate per method on validation couplings (neither training nor held out)
+        Jval = couplings(a.validation, meta["seed"] + 1)
+        grid = tuple(float(v) for v in a.lr_grid.split(","))
+        lr_p, med_p = pick_learning_rate(ans, Jval, np.tile(theta0, (len(Jval), 1)), grid, a.max_iterations,
+                                         a.tolerance, batch=a.batch)
+        log("plain learning rate %g %s" % (lr_p, med_p))
+        lr_w, med_w = pick_learning_rate(ans, Jval, choose_starts(ws, ans, Jval)[0], grid, a.max_iterations,
+                                         a.tolerance, batch=a.batch)
+        log("warm learning rate %g %s" % (lr_w, med_w))
+    rows = evaluate(ans, ws, Jte, theta0, lr_p, lr_w, a.max_iterations, a.tolerance, batch=a.batch, log=log)
     with open(a.out, "w") as f:
         json.dump({"meta": meta, "train_loss": [float(v) for v in hist], "lr_plain": lr_p, "lr_warm": lr_w,
                    "epochs": a.epochs, "max_iterations": a.max_iterations, "tolerance": a.tolerance,
-                   "validation_median_iterations": {"plain": med_p, "warm": med_w}, "rows": rows}, f)
+                   "validation_median_iterations": {"plain": med_p, "warm": med_w}, "offset": a.offset,
+                   "rows": rows}, f)
     ip = [r["iters_plain"] for r in rows]
     iw = [r["iters_warm"] for r in rows]
     print("held-out %d: plain reached %d, warm reached %d; median iterations plain %s warm %s" % (
@@ -387,6 +406,10 @@ def main(argv=None):
     e.add_argument("--max-iterations", type=int, default=1500)
     e.add_argument("--tolerance", type=float, default=1e-10)
     e.add_argument("--validation", type=int, default=16)
+    e.add_argument("--offset", type=int, default=0)
+    e.add_argument("--batch", type=int, default=16)
+    e.add_argument("--lr-plain", type=float, default=None)
+    e.add_argument("--lr-warm", type=float, default=None)
     e.add_argument("--lr-grid", default="0.02,0.05,0.1")
     a = ap.parse_args(argv)
     {"generate": cmd_generate, "evaluate": cmd_evaluate}[a.cmd](a)
