@@ -206,16 +206,27 @@ class WarmStart:
     def __init__(self, net, theta0, scale):
         self.net, self.theta0, self.scale = net, np.asarray(theta0, dtype=np.float64), float(scale)
 
+    # Adam learning rate per stage and its epochs (the recipe the 20-qubit
+    # evaluation used: one stage)
+    SCHEDULE = ((1e-3, 300),)
+
     @classmethod
-    def train(cls, J, X, theta0, hidden=256, epochs=300, batch=256, learning_rate=1e-3, seed=0):
+    def train(cls, J, X, theta0, hidden=256, epochs=None, batch=256, learning_rate=None, seed=0, schedule=None):
         """Fit the network on the GPU to the converged parameters X (rows)
-        of couplings J.  Returns (WarmStart, loss history)."""
+        of couplings J: Adam on the mean squared error, in stages of
+        (learning rate, epochs) (`schedule`, default SCHEDULE; `epochs` and
+        `learning_rate` together give one stage).  Returns (WarmStart, loss
+        history per epoch)."""
         D = np.asarray(X, dtype=np.float64) - np.asarray(theta0)[None, :]
         scale = float(np.std(D)) or 1.0
         net = WarmStartNet(3, hidden, D.shape[1], seed=seed)
-        hist = net.train(np.asarray(J, dtype=np.float32), (D / scale).astype(np.float32), epochs, batch,
-                         learning_rate, seed)
-        return cls(net, theta0, scale), hist
+        if schedule is None:
+            schedule = ((learning_rate, epochs),) if epochs is not None else cls.SCHEDULE
+        xs, ts = np.asarray(J, dtype=np.float32), (D / scale).astype(np.float32)
+        hist = []
+        for k, (lr, ep) in enumerate(schedule):
+            hist.append(net.train(xs, ts, int(ep), batch, float(lr if lr is not None else 1e-3), seed + k))
+        return cls(net, theta0, scale), np.concatenate(hist) if hist else np.zeros(0)
 
     def predict(self, J):
         y = self.net.forward(np.asarray(np.atleast_2d(J), dtype=np.float32)).astype(np.float64)
@@ -360,6 +371,8 @@ def cmd_evaluate(a):
         raise SystemExit("training data incomplete: %d of %d solved" % (solved.sum(), len(J)))
     ans = ry_rz_cz_ring_ansatz(meta["num_qubits"], meta["layers"])
     ws, hist = WarmStart.train(J, X, theta0, epochs=a.epochs, learning_rate=a.lr)
+    log0 = lambda m: print(m, flush=True)
+    log0("trained %d epochs" % len(hist))
     print("trained: loss %.4g -> %.4g" % (hist[0], hist[-1]), flush=True)
     Jte = couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][a.offset:a.offset + a.count]
     log = lambda m: print(m, flush=True)
@@ -378,7 +391,7 @@ def cmd_evaluate(a):
     rows = evaluate(ans, ws, Jte, theta0, lr_p, lr_w, a.max_iterations, a.tolerance, batch=a.batch, log=log)
     with open(a.out, "w") as f:
         json.dump({"meta": meta, "train_loss": [float(v) for v in hist], "lr_plain": lr_p, "lr_warm": lr_w,
-                   "epochs": a.epochs, "max_iterations": a.max_iterations, "tolerance": a.tolerance,
+                   "epochs": a.epochs, "lr": a.lr, "max_iterations": a.max_iterations, "tolerance": a.tolerance,
                    "validation_median_iterations": {"plain": med_p, "warm": med_w}, "offset": a.offset,
                    "rows": rows}, f)
     ip = [r["iters_plain"] for r in rows]
@@ -401,8 +414,8 @@ def main(argv=None):
     e.add_argument("--data", required=True)
     e.add_argument("--out", required=True)
     e.add_argument("--count", type=int, default=256)
-    e.add_argument("--epochs", type=int, default=300)
-    e.add_argument("--lr", type=float, default=1e-3)
+    e.add_argument("--epochs", type=int, default=None, help="one stage of this many epochs (default: the schedule)")
+    e.add_argument("--lr", type=float, default=None)
     e.add_argument("--max-iterations", type=int, default=1500)
     e.add_argument("--tolerance", type=float, default=1e-10)
     e.add_argument("--validation", type=int, default=16)
