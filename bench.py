@@ -64,16 +64,79 @@ def algorithmic_bytes(descs, n, c):
     N = 1 << n
     total = 0
     for g in descs:
-        if g.kind in (abi.RX, abi.RY, abi.RZ, abi.U3, abi.H, abi.MATRIX1, abi.RZZ):
+        if g.kind in (abi.RX, abi.RY, abi.RZ, abi.U3, abi.H, abi.X, abi.Y, abi.MATRIX1, abi.RZZ):
             total += 2 * c * N
-        elif g.kind in (abi.CNOT, abi.SWAP, abi.CRX, abi.CRY, abi.CRZ, abi.PHASE, abi.Z, abi.S, abi.T, abi.X,
-                        abi.Y):
+        elif g.kind in (abi.CNOT, abi.SWAP, abi.CRX, abi.CRY, abi.CRZ, abi.PHASE, abi.Z, abi.S, abi.T):
             total += c * N
         elif g.kind in (abi.CZ, abi.CPHASE, abi.CP, abi.TOFFOLI):
             total += c * N // 2
         else:
             total += 2 * c * N
     return total
+
+
+def line_bytes(g, n, c, line=128):
+    """HBM bytes a gate moves when every 128-byte line holding an amplitude
+    it changes is read and written whole.  A fixed bit (a control, the
+    1-half of a phase gate) whose runs of equal value are shorter than a
+    line no longer halves the lines touched; neither does a SWAP bit."""
+    from logosq_amd import abi
+    N = 1 << n
+    b = [n - 1 - q for q in list(g.qubits)[:abi.ARITY[g.kind]]]
+    fixed = {abi.CNOT: b[:1], abi.CRX: b[:1], abi.CRY: b[:1], abi.CRZ: b[:1], abi.CZ: b, abi.CPHASE: b, abi.CP: b,
+             abi.PHASE: b, abi.Z: b, abi.S: b, abi.T: b, abi.TOFFOLI: b[:2]}.get(g.kind, [])
+    frac = 1.0
+    for p in fixed:
+        if (c << p) >= line:
+            frac /= 2
+    if g.kind == abi.SWAP and all((c << p) >= line for p in b):
+        frac /= 2
+    return int(2 * c * N * frac)
+
+
+def gate_kind_sweep(lq, st, n, c, detail=False):
+    """Per-gate kernels by gate kind on an n-qubit state: every placement of
+    the kind (single-qubit gates on each of the n qubits; two-qubit gates on
+    the n-1 neighbour pairs; Toffoli on the n-2 neighbour triples), each
+    launch timed with HIP events on the library stream.  Rate = algorithmic
+    bytes of the launches (the amplitudes the gate must read and write) over
+    their summed time; the slowest placement is reported beside it, and
+    the rate against line-granular bytes (line_bytes) as well."""
+    from logosq_amd import abi
+    kinds = [("RX", abi.RX, 1, (0.3,)), ("RY", abi.RY, 1, (0.3,)), ("RZ", abi.RZ, 1, (0.3,)), ("H", abi.H, 1, ()),
+             ("X", abi.X, 1, ()), ("Y", abi.Y, 1, ()), ("Phase", abi.PHASE, 1, (0.4,)),
+             ("CNOT", abi.CNOT, 2, ()), ("CZ", abi.CZ, 2, ()), ("CP", abi.CP, 2, (0.5,)),
+             ("SWAP", abi.SWAP, 2, ()), ("CRX", abi.CRX, 2, (0.3,)), ("CRZ", abi.CRZ, 2, (0.3,)),
+             ("RXX", abi.RXX, 2, (0.3,)), ("RZZ", abi.RZZ, 2, (0.3,)), ("Toffoli", abi.TOFFOLI, 3, ())]
+    out = {}
+    for name, kind, ar, th in kinds:
+        # RXX as (q+1, q): the orientation in which the reference applies
+        # the full 4x4 block (src/gates/parameterized.rs:563-575)
+        gates = [abi.make_gate(kind, ([q + 1, q] if kind == abi.RXX else list(range(q, q + ar))), th)
+                 for q in range(n - ar + 1)]
+        tot_b, tot_l, tot_s, worst, rates = 0, 0, 0.0, None, []
+        warm = abi.gate_array(gates[:1])
+        lq._lib.call("lq_execute", st.handle, warm, 1, abi.EXEC_PER_GATE)  # untimed first launch
+        for g in gates:
+            arr = abi.gate_array([g])
+            s = event_time(lq, lambda: lq._lib.call("lq_execute", st.handle, arr, 1, abi.EXEC_PER_GATE))
+            b = algorithmic_bytes([g], n, c)
+            tot_b += b
+            tot_l += line_bytes(g, n, c)
+            tot_s += s
+            r = b / s / 1e9
+            rates.append(round(r, 1))
+            if worst is None or r < worst[0]:
+                worst = (r, list(g.qubits)[:ar])
+        gbs = tot_b / tot_s / 1e9
+        lgbs = tot_l / tot_s / 1e9
+        out[name] = {"GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "line_GBps": round(lgbs, 1), "line_frac": round(lgbs / HBM_PEAK_GBS, 4),
+                     "bytes_per_launch": algorithmic_bytes(gates[:1], n, c), "launches": len(gates),
+                     "slowest_GBps": round(worst[0], 1), "slowest_qubits": worst[1]}
+        if detail:
+            out[name]["GBps_by_first_qubit"] = rates
+    return out
 
 
 def time_kernels(lq, st, gates, reps=1):
@@ -373,6 +436,10 @@ def main():
     t_cn = event_time(lq, lambda: lq._lib.call("lq_execute", st.handle, cn_arr, len(cn), abi.EXEC_PER_GATE))
     rx_gbs = 2 * c * N * len(rx) / t_rx / 1e9
     cn_gbs = c * N * len(cn) / t_cn / 1e9
+    by_kind = None
+    if not a.no_extra:
+        print("bench: per-gate kernels by kind", file=sys.stderr, flush=True)
+        by_kind = gate_kind_sweep(lq, st, a.qubits, c)
     # the other execution mode of the same circuit
     other = abi.EXEC_FUSED if mode == abi.EXEC_PER_GATE else abi.EXEC_PER_GATE
     oname = "fused" if other == abi.EXEC_FUSED else "per_gate"
@@ -437,7 +504,7 @@ def main():
                    "final_norm_sqr": norm2,
                    "per_gate_kernels": {"rx_pair_GBps": round(rx_gbs, 1), "rx_frac": round(rx_gbs / HBM_PEAK_GBS, 4),
                                         "cnot_GBps": round(cn_gbs, 1), "cnot_frac": round(cn_gbs / HBM_PEAK_GBS, 4),
-                                        "c64": c64},
+                                        "c64": c64, "by_kind": by_kind},
                    "fused_execution": fused},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.mode),

@@ -2,16 +2,19 @@
 // (src/gates/single_qubit.rs, two_qubit.rs, three_qubit.rs, parameterized.rs,
 // src/gates/mod.rs:77-93).
 //
-// Layout: amplitudes contiguous, qubit q at bit (n-1-q).  A wavefront (64
-// lanes) owns a "chunk" of 64 consecutive amplitudes, one per lane, so every
-// global load and store is a coalesced 1 KiB (c128) / 512 B (c64) access.
-//   - target bit >= 6: the partner amplitude lives in another chunk; the wave
-//     loads both chunks and updates the pair in registers;
-//   - target bit <  6: the partner lives in lane (lane ^ 2^t) of the same
-//     chunk and is fetched with a cross-lane shuffle.
-// Fixed bits (controls, the 1-half of phase gates) at or above bit 6 are
-// deposited into the chunk index, so chunks that the gate does not touch are
-// never read; fixed bits below 6 predicate the stores lane by lane.
+// Layout: amplitudes contiguous, qubit q at bit (n-1-q).  A gate's bits split
+// into fixed bits (controls and the 1-half of phase gates: always 1 in the
+// amplitudes it touches), enumerated bits (a lane holds every value of
+// them) and free bits.  Lane l of a wavefront covers the 6 lowest free bits,
+// the work unit the remaining ones, so each global load or store of a
+// wavefront reads the densest run of amplitudes the gate touches (1 KiB of
+// consecutive c128 amplitudes when no fixed bit lies below bit 6) and
+// amplitudes the gate leaves alone are never read.
+//   - pair target beyond the 6 lowest free bits: enumerated; the lane loads
+//     both halves of its pair and updates them in registers;
+//   - pair target among them: the partner lives in lane (lane ^ 2^tl) of the
+//     same wavefront and is fetched with a cross-lane shuffle.
+// SWAP and 4x4 blocks enumerate both of their bits.
 #include <cmath>
 #include <cstring>
 
@@ -55,17 +58,30 @@ __device__ __forceinline__ void gstore(V* p, V a) {
     else *reinterpret_cast<N*>(p) = v;
 }
 
-template <typename T> struct ChunkBits {
-    int pos[4];  // fixed positions in chunk-index space (amp bit - 6), ascending
-    int count;
+// Bit positions that a launch's lanes and work units skip, ascending: the
+// fixed bits (value 1 in every amplitude the gate touches: controls, the
+// 1-half of phase gates) and the bits whose values a lane enumerates itself
+// (a pair kernel's high target, both bits of a SWAP or 4x4 block).  The
+// remaining "free" bits are numbered from the bottom: lane l covers the 6
+// lowest free bits and the work-unit index the rest, so a wavefront's 64
+// amplitudes are the densest run of amplitudes the gate must touch.
+constexpr int kMaxSkip = 5;
+struct SkipBits {
+    // seg[i]: the bits of a compressed free-bit index that move up by i
+    // places (i skipped positions lie below them)
+    uint64_t seg[kMaxSkip + 1];
+    uint64_t ones;   // the skipped fixed bits (all 1)
+    uint64_t dense;  // fixed bits kept among the lane bits (see geometry())
 };
 
-__device__ __forceinline__ uint64_t deposit(uint64_t k, const int* pos, int count) {
-    for (int i = 0; i < count; ++i) {
-        int p = pos[i];
-        k = ((k >> p) << (p + 1)) | (k & ((1ull << p) - 1ull));
-    }
-    return k;
+// Compressed free-bit index -> amplitude index with the skipped bits 0.  A
+// bit-wise map, so the lane and unit parts can be spread separately and
+// OR-ed; branch-free, and on the scalar unit for the wave-uniform unit part.
+__device__ __forceinline__ uint64_t spread(uint64_t k, const SkipBits& s) {
+    uint64_t r = k & s.seg[0];
+#pragma unroll
+    for (int i = 1; i <= kMaxSkip; ++i) r |= (k & s.seg[i]) << i;
+    return r;
 }
 
 template <typename V>
@@ -101,77 +117,80 @@ __device__ __forceinline__ void pair_op(const Mat2<T>& m, V a0, V a1, V& o0, V& 
 
 constexpr int kOpHadamard = 3;
 
-// Target bit >= 6: each work unit is a pair of chunks (target bit 0 / 1).
+// Wave index and wave count of a launch (U units per wave, strided by W).
+// Wave indices are wave-uniform: readfirstlane puts them (and the unit
+// address arithmetic) on the scalar unit.  Every launch uses kThreads.
+#define LQ_WAVE_IDS                                                                          \
+    const uint32_t lane = threadIdx.x & (kWave - 1);                                         \
+    const uint64_t w = (uint64_t)__builtin_amdgcn_readfirstlane(                             \
+        blockIdx.x * (uint32_t)(kThreads / kWave) + (threadIdx.x >> kLaneBits));              \
+    const uint64_t W = (uint64_t)gridDim.x * (kThreads / kWave);
+#define LQ_UNIT(j) (w + (uint64_t)(j) * W)
+
+// Pair kernel, target among the skipped bits: each lane holds both halves
+// of its pair (a0 at idx, a1 at idx + tstride).
 template <typename V, typename T, int OP, int U, bool NTM>
-__global__ void __launch_bounds__(kThreads) k_pair_hi(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
-                                                      uint64_t chunk_or, uint64_t tstride, uint32_t lowctrl,
-                                                      Mat2<T> m) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> kLaneBits;
-    const uint64_t W = ((uint64_t)gridDim.x * blockDim.x) >> kLaneBits;
-    const bool act = (lane & lowctrl) == lowctrl;
+__global__ void __launch_bounds__(kThreads) k_pair_hi(V* __restrict__ psi, uint64_t units, SkipBits sk,
+                                                      uint64_t tstride, uint32_t lane_limit, Mat2<T> m) {
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    const bool on = (loff & sk.dense) == sk.dense;
     V a0[U], a1[U];
     uint64_t idx[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        uint64_t u = w + (uint64_t)j * W;
+        const uint64_t u = LQ_UNIT(j);
         idx[j] = ~0ull;
-        if (u < units) {
-            uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
-            idx[j] = (c << kLaneBits) | lane;
+        if (u < units && live) {
+            idx[j] = spread(u << kLaneBits, sk) | loff;
             a0[j] = gload<NTM>(psi + idx[j]);
             a1[j] = gload<NTM>(psi + idx[j] + tstride);
         }
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        if (idx[j] != ~0ull && act) {
-            V o0, o1;
-            pair_op<OP, V, T>(m, a0[j], a1[j], o0, o1);
+        if (idx[j] != ~0ull) {
+            V o0 = a0[j], o1 = a1[j];
+            if (on) pair_op<OP, V, T>(m, a0[j], a1[j], o0, o1);
             gstore<NTM>(psi + idx[j], o0);
             gstore<NTM>(psi + idx[j] + tstride, o1);
         }
     }
 }
 
-// Target bit < 6: each work unit is one chunk; partner via lane shuffle.
+// Pair kernel, target among the 6 lowest free bits (lane bit tl): one
+// amplitude per lane, partner through a cross-lane shuffle.
 template <typename V, typename T, int OP, int U, bool NTM>
-__global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
-                                                      uint64_t chunk_or, int tbit, uint32_t lowctrl,
+__global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint64_t units, SkipBits sk, int tl,
                                                       uint32_t lane_limit, Mat2<T> m) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> kLaneBits;
-    const uint64_t W = ((uint64_t)gridDim.x * blockDim.x) >> kLaneBits;
-    const bool in_range = lane < lane_limit;
-    const bool act = in_range && ((lane & lowctrl) == lowctrl);
-    const bool hi_half = (lane >> tbit) & 1u;
-    const int pmask = 1 << tbit;
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    const bool on = (loff & sk.dense) == sk.dense;  // the partner lane agrees
+    const bool hi_half = (lane >> tl) & 1u;
+    const int pmask = 1 << tl;
     V a[U];
     uint64_t idx[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        uint64_t u = w + (uint64_t)j * W;  // wave-uniform
+        const uint64_t u = LQ_UNIT(j);  // wave-uniform
         idx[j] = ~0ull;
         a[j].x = 0;
         a[j].y = 0;
         if (u < units) {
-            uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
-            idx[j] = (c << kLaneBits) | lane;
-            if (in_range) a[j] = gload<NTM>(psi + idx[j]);
+            idx[j] = spread(u << kLaneBits, sk) | loff;
+            if (live) a[j] = gload<NTM>(psi + idx[j]);
         }
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        V p = shfl_xor_c(a[j], pmask);  // all lanes participate
-        if (idx[j] != ~0ull && act) {
+        V p = shfl_xor_c(a[j], pmask);  // every lane takes part
+        if (idx[j] != ~0ull && live) {
             V o0, o1;
-            if (hi_half) {
-                pair_op<OP, V, T>(m, p, a[j], o0, o1);
-                gstore<NTM>(psi + idx[j], o1);
-            } else {
-                pair_op<OP, V, T>(m, a[j], p, o0, o1);
-                gstore<NTM>(psi + idx[j], o0);
-            }
+            if (hi_half) pair_op<OP, V, T>(m, p, a[j], o0, o1);
+            else pair_op<OP, V, T>(m, a[j], p, o0, o1);
+            gstore<NTM>(psi + idx[j], !on ? a[j] : hi_half ? o1 : o0);
         }
     }
 }
@@ -179,71 +198,98 @@ __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint6
 // Diagonal phases: every amplitude whose fixed bits are all 1 is multiplied
 // by d[bit(sel0) | bit(sel1) << 1].
 template <typename V, typename T, int U, bool NTM>
-__global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t units, ChunkBits<T> fb,
-                                                   uint64_t chunk_or, uint32_t lowfixed, uint32_t lane_limit,
-                                                   int sel0, int sel1, Diag4<T> d) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> kLaneBits;
-    const uint64_t W = ((uint64_t)gridDim.x * blockDim.x) >> kLaneBits;
-    const bool act = lane < lane_limit && ((lane & lowfixed) == lowfixed);
+__global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t units, SkipBits sk,
+                                                   uint32_t lane_limit, int sel0, int sel1, Diag4<T> d) {
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    const bool on = (loff & sk.dense) == sk.dense;
     V a[U];
     uint64_t idx[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-        uint64_t u = w + (uint64_t)j * W;
+        const uint64_t u = LQ_UNIT(j);
         idx[j] = ~0ull;
-        if (u < units && act) {
-            uint64_t c = deposit(u, fb.pos, fb.count) | chunk_or;
-            idx[j] = (c << kLaneBits) | lane;
+        if (u < units && live) {
+            idx[j] = spread(u << kLaneBits, sk) | loff;
             a[j] = gload<NTM>(psi + idx[j]);
         }
     }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
         if (idx[j] != ~0ull) {
-            uint64_t i = idx[j];
+            const uint64_t i = idx[j];
             int k = 0;
             if (sel0 >= 0) k |= (int)((i >> sel0) & 1ull);
             if (sel1 >= 0) k |= (int)((i >> sel1) & 1ull) << 1;
             const T pr = k == 3 ? d.d[6] : k == 2 ? d.d[4] : k == 1 ? d.d[2] : d.d[0];
             const T pi = k == 3 ? d.d[7] : k == 2 ? d.d[5] : k == 1 ? d.d[3] : d.d[1];
-            gstore<NTM>(psi + i, cmul(pr, pi, a[j]));
+            gstore<NTM>(psi + i, on ? cmul(pr, pi, a[j]) : a[j]);
         }
     }
 }
 
-// SWAP (two_qubit.rs:40-62): swap |..1..0..> with |..0..1..> for bits (b1, b2).
-template <typename V>
-__global__ void __launch_bounds__(kThreads) k_swap2(V* __restrict__ psi, uint64_t count, int lo, int hi,
-                                                    uint64_t m1, uint64_t m2) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
-        uint64_t i = insert_zero(insert_zero(k, lo), hi) | m1;
-        uint64_t j = i ^ m1 ^ m2;
-        V x = psi[i];
-        V y = psi[j];
-        psi[i] = y;
-        psi[j] = x;
+// SWAP (two_qubit.rs:40-62) of bits (ba, bb), both skipped: each lane
+// exchanges |..ba=1..bb=0..> with |..ba=0..bb=1..>; the 00 and 11 amplitudes
+// are never read.
+template <typename V, int U, bool NTM>
+__global__ void __launch_bounds__(kThreads) k_swap2(V* __restrict__ psi, uint64_t units, SkipBits sk,
+                                                    uint32_t lane_limit, uint64_t ma, uint64_t mb) {
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    V x[U], y[U];
+    uint64_t idx[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = LQ_UNIT(j);
+        idx[j] = ~0ull;
+        if (u < units && live) {
+            idx[j] = spread(u << kLaneBits, sk) | loff;
+            x[j] = gload<NTM>(psi + (idx[j] | ma));
+            y[j] = gload<NTM>(psi + (idx[j] | mb));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        if (idx[j] != ~0ull) {
+            gstore<NTM>(psi + (idx[j] | ma), y[j]);
+            gstore<NTM>(psi + (idx[j] | mb), x[j]);
+        }
     }
 }
 
-// General 4x4 block on bits (ba, bb), sub-index (bit(ba) << 1) | bit(bb).
-template <typename V, typename T>
-__global__ void __launch_bounds__(kThreads) k_quad(V* __restrict__ psi, uint64_t count, int lo, int hi,
-                                                   uint64_t ma, uint64_t mb, Mat4<T> m) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
-        uint64_t i = insert_zero(insert_zero(k, lo), hi);
-        uint64_t ix[4] = {i, i | mb, i | ma, i | ma | mb};
-        V a[4];
+// General 4x4 block on bits (ba, bb), both skipped; sub-index
+// (bit(ba) << 1) | bit(bb).  Each lane holds its four amplitudes.
+template <typename V, typename T, int U, bool NTM>
+__global__ void __launch_bounds__(kThreads) k_quad(V* __restrict__ psi, uint64_t units, SkipBits sk,
+                                                   uint32_t lane_limit, uint64_t ma, uint64_t mb, Mat4<T> m) {
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    const uint64_t off[4] = {0, mb, ma, ma | mb};
+    V a[U][4];
+    uint64_t idx[U];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) a[s] = psi[ix[s]];
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = LQ_UNIT(j);
+        idx[j] = ~0ull;
+        if (u < units && live) {
+            idx[j] = spread(u << kLaneBits, sk) | loff;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            V acc = cmul(m.m[8 * r], m.m[8 * r + 1], a[0]);
+            for (int s = 0; s < 4; ++s) a[j][s] = gload<NTM>(psi + (idx[j] | off[s]));
+        }
+    }
 #pragma unroll
-            for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m.m[8 * r + 2 * c], m.m[8 * r + 2 * c + 1], a[c]));
-            psi[ix[r]] = acc;
+    for (int j = 0; j < U; ++j) {
+        if (idx[j] != ~0ull) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                V acc = cmul(m.m[8 * r], m.m[8 * r + 1], a[j][0]);
+#pragma unroll
+                for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m.m[8 * r + 2 * c], m.m[8 * r + 2 * c + 1], a[j][c]));
+                gstore<NTM>(psi + (idx[j] | off[r]), acc);
+            }
         }
     }
 }
@@ -282,25 +328,63 @@ __global__ void __launch_bounds__(kThreads) k_dense(const double2* __restrict__ 
 static int g_unroll = 2;
 static bool g_nontemporal = true;
 
-struct Split {
-    ChunkBits<double> fb{};   // chunk-space fixed positions (positions only)
-    uint64_t chunk_or = 0;    // chunk-space value of the fixed bits (all 1)
-    uint32_t lowmask = 0;     // lane-space fixed bits (must be 1)
-    int chunk_bits = 0;       // number of chunk-index bits (n - 6, >= 0)
+// Fixed bits whose runs of equal value are shorter than this many bytes
+// stay among the lane bits (a lane whose copy of them is not all 1 stores
+// its amplitude back unchanged): HBM writes that cover only part of such a
+// short run cost more than writing the whole run.
+#ifndef LQ_DENSE_BYTES
+#define LQ_DENSE_BYTES 128
+#endif
+constexpr uint64_t kDenseBytes = LQ_DENSE_BYTES;
+
+// True when bit b of an amplitude index splits the state into runs shorter
+// than kDenseBytes.
+static inline bool short_run(int b, size_t amp_bytes) { return ((uint64_t)amp_bytes << b) < kDenseBytes; }
+
+// The launch geometry of one gate: skipped bits, the number of free bits
+// (n - skipped), the work units (each 64 free-bit values) and the live
+// lanes (fewer than 64 only for tiny states).
+struct Geometry {
+    SkipBits sk{};
+    bool ok = true;  // at most kMaxSkip skipped bits
+    int free_bits = 0;
+    uint64_t units = 1;
+    uint32_t lane_limit = kWave;
 };
 
-// Split a set of amplitude bits that must equal 1 (`ones`) and bits that are
-// enumerated separately (`zeros`, e.g. a high target) into chunk/lane parts.
-static Split split_bits(int n, uint64_t ones, uint64_t zeros) {
-    Split s;
-    s.chunk_bits = n > kLaneBits ? n - kLaneBits : 0;
-    uint64_t all = ones | zeros;
-    s.fb.count = 0;
-    for (int b = kLaneBits; b < n; ++b)
-        if (all & (1ull << b)) s.fb.pos[s.fb.count++] = b - kLaneBits;
-    s.chunk_or = (ones >> kLaneBits);
-    s.lowmask = (uint32_t)(ones & ((1ull << kLaneBits) - 1ull));
-    return s;
+static uint64_t dense_bits(uint64_t ones, size_t amp_bytes) {
+    uint64_t d = 0;
+    for (int b = 0; b < 64 && short_run(b, amp_bytes); ++b) d |= ones & (1ull << b);
+    return d;
+}
+
+static Geometry geometry(int n, uint64_t ones, uint64_t skip, size_t amp_bytes) {
+    Geometry g;
+    const uint64_t dense = dense_bits(ones, amp_bytes);
+    const uint64_t all = (ones & ~dense) | skip;
+    for (int i = 0; i <= kMaxSkip; ++i) g.sk.seg[i] = 0;
+    int below = 0, j = 0;  // skipped positions passed; compressed bit index
+    for (int b = 0; b < n; ++b) {
+        if (all & (1ull << b)) {
+            if (++below > kMaxSkip) {
+                g.ok = false;
+                return g;
+            }
+        } else {
+            g.sk.seg[below] |= 1ull << j++;
+        }
+    }
+    g.sk.ones = ones & ~dense;
+    g.sk.dense = dense;
+    g.free_bits = j;
+    if (g.free_bits >= kLaneBits) {
+        g.units = 1ull << (g.free_bits - kLaneBits);
+        g.lane_limit = kWave;
+    } else {
+        g.units = 1;
+        g.lane_limit = 1u << g.free_bits;
+    }
+    return g;
 }
 
 static unsigned blocks_for_units(uint64_t units, int unroll) {
@@ -310,32 +394,31 @@ static unsigned blocks_for_units(uint64_t units, int unroll) {
     return (unsigned)blocks;
 }
 
-template <typename V, typename T, int OP, int U, bool NTM>
-static void pair_go(V* psi, bool hi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, int tbit,
-                    uint32_t lowmask, uint32_t lane_limit, const Mat2<T>& m) {
-    unsigned g = blocks_for_units(units, U);
-    if (hi)
-        hipLaunchKernelGGL((k_pair_hi<V, T, OP, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb,
-                           chunk_or, 1ull << tbit, lowmask, m);
-    else
-        hipLaunchKernelGGL((k_pair_lo<V, T, OP, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb,
-                           chunk_or, tbit, lowmask, lane_limit, m);
+// Calls F.template operator()<U, NT>() for the configured variant.
+template <typename F>
+static void with_variant(F&& f) {
+    if (g_nontemporal) {
+        if (g_unroll == 2) f.template operator()<2, true>();
+        else if (g_unroll == 8) f.template operator()<8, true>();
+        else f.template operator()<4, true>();
+    } else {
+        if (g_unroll == 2) f.template operator()<2, false>();
+        else if (g_unroll == 8) f.template operator()<8, false>();
+        else f.template operator()<4, false>();
+    }
 }
 
 template <typename V, typename T, int OP>
-static void pair_pick(V* psi, bool hi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, int tbit,
-                      uint32_t lowmask, uint32_t lane_limit, const Mat2<T>& m) {
-#define LQ_PAIR(U, NTM) pair_go<V, T, OP, U, NTM>(psi, hi, units, fb, chunk_or, tbit, lowmask, lane_limit, m)
-    if (g_nontemporal) {
-        if (g_unroll == 2) LQ_PAIR(2, true);
-        else if (g_unroll == 8) LQ_PAIR(8, true);
-        else LQ_PAIR(4, true);
-    } else {
-        if (g_unroll == 2) LQ_PAIR(2, false);
-        else if (g_unroll == 8) LQ_PAIR(8, false);
-        else LQ_PAIR(4, false);
-    }
-#undef LQ_PAIR
+static void pair_launch(V* psi, const Geometry& g, bool hi, int tsel, const Mat2<T>& m) {
+    with_variant([&]<int U, bool NT>() {
+        const unsigned blocks = blocks_for_units(g.units, U);
+        if (hi)
+            hipLaunchKernelGGL((k_pair_hi<V, T, OP, U, NT>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
+                               g.sk, 1ull << tsel, g.lane_limit, m);
+        else
+            hipLaunchKernelGGL((k_pair_lo<V, T, OP, U, NT>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
+                               g.sk, tsel, g.lane_limit, m);
+    });
 }
 
 template <typename T, typename V>
@@ -344,18 +427,23 @@ static int launch_pair_t(lq_state* st, int tbit, uint64_t ctrl, const double md[
     for (int i = 0; i < 8; ++i) m.m[i] = (T)md[i];
     const int n = st->n;
     V* psi = (V*)st->d_amps;
-    const bool hi = tbit >= kLaneBits;
-    Split s = split_bits(n, ctrl, hi ? (1ull << tbit) : 0);
-    ChunkBits<T> fb;
-    fb.count = s.fb.count;
-    for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
-    uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
-    uint32_t lane_limit = n >= kLaneBits ? (uint32_t)kWave : (1u << n);
+    // The target's rank among the free bits (skipped controls left out):
+    // below 6 the partner is in the same wavefront, reached by a shuffle on
+    // lane bit tl.
+    const uint64_t skipped = ctrl & ~dense_bits(ctrl, st->amp_bytes);
+    const int tl = tbit - __builtin_popcountll(skipped & ((1ull << tbit) - 1ull));
+    const bool hi = tl >= kLaneBits;
+    const Geometry g = geometry(n, ctrl, hi ? (1ull << tbit) : 0, st->amp_bytes);
+    const int tsel = hi ? tbit : tl;
+    if (!g.ok) {
+        set_error("gate fixes more than %d bits", kMaxSkip);
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
     switch (op) {
-        case kOpMatrix: pair_pick<V, T, kOpMatrix>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
-        case kOpSwap: pair_pick<V, T, kOpSwap>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
-        case kOpNegY: pair_pick<V, T, kOpNegY>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
-        default: pair_pick<V, T, kOpHadamard>(psi, hi, units, fb, s.chunk_or, tbit, s.lowmask, lane_limit, m); break;
+        case kOpMatrix: pair_launch<V, T, kOpMatrix>(psi, g, hi, tsel, m); break;
+        case kOpSwap: pair_launch<V, T, kOpSwap>(psi, g, hi, tsel, m); break;
+        case kOpNegY: pair_launch<V, T, kOpNegY>(psi, g, hi, tsel, m); break;
+        default: pair_launch<V, T, kOpHadamard>(psi, g, hi, tsel, m); break;
     }
     LQ_LAUNCH_CHECK();
     return LQ_OK;
@@ -366,36 +454,20 @@ int launch_pair(lq_state* st, int tbit, uint64_t ctrl_mask, const double m[8], i
     return launch_pair_t<float, float2>(st, tbit, ctrl_mask, m, op);
 }
 
-template <typename V, typename T, int U, bool NTM>
-static void diag_go(V* psi, uint64_t units, const ChunkBits<T>& fb, uint64_t chunk_or, uint32_t lowmask,
-                    uint32_t lane_limit, int sel0, int sel1, const Diag4<T>& d) {
-    unsigned g = blocks_for_units(units, U);
-    hipLaunchKernelGGL((k_diag<V, T, U, NTM>), dim3(g), dim3(kThreads), 0, stream(), psi, units, fb, chunk_or,
-                       lowmask, lane_limit, sel0, sel1, d);
-}
-
 template <typename T, typename V>
 static int launch_diag_t(lq_state* st, uint64_t fixed, int sel0, int sel1, const double dd[8]) {
     Diag4<T> d;
     for (int i = 0; i < 8; ++i) d.d[i] = (T)dd[i];
-    Split s = split_bits(st->n, fixed, 0);
-    ChunkBits<T> fb;
-    fb.count = s.fb.count;
-    for (int i = 0; i < 4; ++i) fb.pos[i] = s.fb.pos[i];
-    uint64_t units = (1ull << s.chunk_bits) >> s.fb.count;
-    uint32_t lane_limit = st->n >= kLaneBits ? (uint32_t)kWave : (1u << st->n);
+    const Geometry g = geometry(st->n, fixed, 0, st->amp_bytes);
     V* psi = (V*)st->d_amps;
-#define LQ_DIAG(U, NTM) diag_go<V, T, U, NTM>(psi, units, fb, s.chunk_or, s.lowmask, lane_limit, sel0, sel1, d)
-    if (g_nontemporal) {
-        if (g_unroll == 2) LQ_DIAG(2, true);
-        else if (g_unroll == 8) LQ_DIAG(8, true);
-        else LQ_DIAG(4, true);
-    } else {
-        if (g_unroll == 2) LQ_DIAG(2, false);
-        else if (g_unroll == 8) LQ_DIAG(8, false);
-        else LQ_DIAG(4, false);
+    if (!g.ok) {
+        set_error("gate fixes more than %d bits", kMaxSkip);
+        return LQ_ERR_INVALID_ARGUMENT;
     }
-#undef LQ_DIAG
+    with_variant([&]<int U, bool NT>() {
+        hipLaunchKernelGGL((k_diag<V, T, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0, stream(),
+                           psi, g.units, g.sk, g.lane_limit, sel0, sel1, d);
+    });
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
@@ -415,42 +487,55 @@ int set_gate_kernel_config(int unroll, int nontemporal) {
     return LQ_OK;
 }
 
-static unsigned grid_stride_blocks(uint64_t count) {
-    uint64_t b = (count + kThreads - 1) / kThreads;
-    if (b > 65536) b = 65536;
-    if (b < 1) b = 1;
-    return (unsigned)b;
+template <typename V>
+static void swap2_t(lq_state* st, int ba, int bb) {
+    const Geometry g = geometry(st->n, 0, (1ull << ba) | (1ull << bb), st->amp_bytes);
+    V* psi = (V*)st->d_amps;
+    with_variant([&]<int U, bool NT>() {
+        hipLaunchKernelGGL((k_swap2<V, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0, stream(),
+                           psi, g.units, g.sk, g.lane_limit, 1ull << ba, 1ull << bb);
+    });
 }
 
+int launch_quad(lq_state* st, int ba, int bb, const double m[32]);
+
 int launch_swap2(lq_state* st, int b1, int b2) {
-    int lo = b1 < b2 ? b1 : b2, hi = b1 < b2 ? b2 : b1;
-    uint64_t count = st->dim >> 2;
-    unsigned g = grid_stride_blocks(count);
-    if (st->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_swap2<double2>, dim3(g), dim3(kThreads), 0, stream(), (double2*)st->d_amps, count, lo,
-                           hi, 1ull << b1, 1ull << b2);
-    else
-        hipLaunchKernelGGL(k_swap2<float2>, dim3(g), dim3(kThreads), 0, stream(), (float2*)st->d_amps, count, lo, hi,
-                           1ull << b1, 1ull << b2);
+    if (short_run(b1, st->amp_bytes) || short_run(b2, st->amp_bytes)) {
+        // The 01 and 10 amplitudes form short runs: write every amplitude
+        // (the 4x4 permutation, exact: products with 1 and 0 only).
+        double pm[32] = {0};
+        pm[0] = pm[8 * 1 + 2 * 2] = pm[8 * 2 + 2 * 1] = pm[8 * 3 + 2 * 3] = 1.0;
+        return launch_quad(st, b1, b2, pm);
+    }
+    if (st->dtype == LQ_C128) swap2_t<double2>(st, b1, b2);
+    else swap2_t<float2>(st, b1, b2);
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
 
+template <typename V, typename T>
+static void quad_t(lq_state* st, int ba, int bb, const double* md) {
+    Mat4<T> m;
+    for (int i = 0; i < 32; ++i) m.m[i] = (T)md[i];
+    const Geometry g = geometry(st->n, 0, (1ull << ba) | (1ull << bb), st->amp_bytes);
+    V* psi = (V*)st->d_amps;
+    // The 4x4 kernel reads all four amplitudes of a lane's group; with a bit
+    // below the wave's span its four loads share cache lines, which the
+    // cached (temporal) form fetches once.
+    with_variant([&]<int U, bool NT>() {
+        const unsigned blocks = blocks_for_units(g.units, U);
+        if (NT && ba >= kLaneBits && bb >= kLaneBits)
+            hipLaunchKernelGGL((k_quad<V, T, U, true>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
+                               g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
+        else
+            hipLaunchKernelGGL((k_quad<V, T, U, false>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
+                               g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
+    });
+}
+
 int launch_quad(lq_state* st, int ba, int bb, const double m[32]) {
-    int lo = ba < bb ? ba : bb, hi = ba < bb ? bb : ba;
-    uint64_t count = st->dim >> 2;
-    unsigned g = grid_stride_blocks(count);
-    if (st->dtype == LQ_C128) {
-        Mat4<double> mm;
-        for (int i = 0; i < 32; ++i) mm.m[i] = m[i];
-        hipLaunchKernelGGL((k_quad<double2, double>), dim3(g), dim3(kThreads), 0, stream(), (double2*)st->d_amps,
-                           count, lo, hi, 1ull << ba, 1ull << bb, mm);
-    } else {
-        Mat4<float> mm;
-        for (int i = 0; i < 32; ++i) mm.m[i] = (float)m[i];
-        hipLaunchKernelGGL((k_quad<float2, float>), dim3(g), dim3(kThreads), 0, stream(), (float2*)st->d_amps, count,
-                           lo, hi, 1ull << ba, 1ull << bb, mm);
-    }
+    if (st->dtype == LQ_C128) quad_t<double2, double>(st, ba, bb, m);
+    else quad_t<float2, float>(st, ba, bb, m);
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
