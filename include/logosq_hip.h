@@ -219,6 +219,30 @@ int lq_autotune(int num_qubits, int dtype, char* report, size_t report_len);
 int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, int count, int tile_bits, int low_bits,
                   int* gate_order, int* pass_index, int* num_passes);
 
+/* ---- Noise channels (src/noise/mod.rs:10-380) ---------------------------
+ * Circuit::execute applies every noise model after every operation
+ * (src/circuits/mod.rs:359-371).  `qubits` are the reference's qubit numbers
+ * used as it uses them in these loops: bit `1 << qubit` of the basis index
+ * (mod.rs:129,197,343-376), not bit n-1-qubit as for gates.  Indices >= n
+ * are rejected (InvalidQubitIndex); probabilities outside [0, 1] are
+ * rejected (InvalidArgument). */
+/* AmplitudeDampingNoise::apply (mod.rs:110-147): out = sum over the listed
+ * qubits of (K0 + K1) psi with K0 = |0><0| + sqrt(1-gamma)|1><1| and
+ * K1 = sqrt(gamma)|0><1|, then State::normalize.  An empty list gives the
+ * zero vector, as the reference's does. */
+int lq_noise_amplitude_damping(lq_state* st, const int* qubits, int count, double gamma);
+/* PhaseDampingNoise::apply (mod.rs:173-206): the last listed qubit's |1>
+ * amplitudes times sqrt(1-lambda) (the reference resets its output from
+ * psi for every qubit), then State::normalize; empty list: zero vector. */
+int lq_noise_phase_damping(lq_state* st, const int* qubits, int count, double lambda);
+/* DepolarizingNoise::apply (mod.rs:63-86): per listed qubit, when a uniform
+ * u < p, a second uniform picks X, Y (the reference's i / -i form) or Z
+ * (floor(3u), capped at 2).  Uniforms are the top 53 bits of SplitMix64
+ * outputs from *rng_state, which is advanced; *errors (nullable) gets the
+ * number of errors applied. */
+int lq_noise_depolarizing(lq_state* st, const int* qubits, int count, double p, uint64_t* rng_state,
+                          int* errors);
+
 /* ---- Observables (src/optimization/observable.rs) ----------------------- */
 /* PauliObservable::expectation (observable.rs:228-231) =
  * sum_t coeff[t] * PauliTerm::expectation (observable.rs:95-147).

@@ -15,6 +15,9 @@ from .optimization import (Pauli, PauliTerm, PauliObservable, Observable, Ansatz
                            EntanglingPattern, ParameterShift, AdjointGradient, FiniteDifference, GradientDescent, Adam, VQE,
                            heisenberg_ring, ry_rz_cz_ring_ansatz, vqe_adam_batch)
 from . import algorithms
+from . import noise
+from .noise import (NoiseModel, NoiseParams, DepolarizingNoise, AmplitudeDampingNoise, PhaseDampingNoise,
+                    ThermalRelaxationNoise, CompositeNoise)
 from .batch import batch_expectations
 
 __version__ = "0.1.0"

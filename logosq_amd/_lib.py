@@ -51,6 +51,9 @@ SIGNATURES = {
     "lq_bit_probabilities": [c_void_p, P_double],
     "lq_measure_qubit": [c_void_p, c_int, c_double, P_int],
     "lq_sample_shots": [c_void_p, c_u64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)],
+    "lq_noise_amplitude_damping": [c_void_p, P_int, c_int, c_double],
+    "lq_noise_phase_damping": [c_void_p, P_int, c_int, c_double],
+    "lq_noise_depolarizing": [c_void_p, P_int, c_int, c_double, ctypes.POINTER(ctypes.c_uint64), P_int],
     "lq_apply_gate": [c_void_p, P_gate],
     "lq_apply_1q": [c_void_p, c_int, P_double],
     "lq_apply_ctrl_1q": [c_void_p, P_int, c_int, c_int, P_double],

@@ -1,6 +1,8 @@
 """Algorithms on the GPU path: QFT circuits (src/algorithms/qft.rs:10-46),
-the seeded brickwork circuit modelled on rqc::generate_supremacy_circuit
-(src/algorithms/rqc.rs:236-268), and the documented input generators."""
+XYZ Heisenberg Trotter evolution and energy
+(src/algorithms/xyz_heisenberg.rs:8-162), the seeded brickwork circuit
+modelled on rqc::generate_supremacy_circuit (src/algorithms/rqc.rs:236-268),
+and the documented input generators."""
 import math
 
 import numpy as np
@@ -70,6 +72,69 @@ def inverse_qft_circuit(num_qubits):
             c.controlled_phase(i, j, -math.pi / float(1 << (j - i)))
         c.h(i)
     return c
+
+
+class HeisenbergParameters:
+    """HeisenbergParameters (src/algorithms/xyz_heisenberg.rs:8-28), with the
+    reference's defaults."""
+
+    def __init__(self, jx=1.0, jy=1.0, jz=1.0, external_field=0.0, time_steps=100, dt=0.01):
+        self.jx, self.jy, self.jz = float(jx), float(jy), float(jz)
+        self.external_field = float(external_field)
+        self.time_steps = int(time_steps)
+        self.dt = float(dt)
+
+
+def xyz_heisenberg_circuit(num_qubits, params):
+    """xyz_heisenberg::create_circuit (xyz_heisenberg.rs:31-61): per Trotter
+    step RXX(i, i+1, 2 Jx dt) along the open chain, then RYY(2 Jy dt), then
+    RZZ(2 Jz dt), then RZ(i, h dt) on every qubit when h != 0.  RXX/RYY keep
+    the reference's orientation rule (qubit1 < qubit2 here, so their 01<->10
+    block is left alone, src/gates/parameterized.rs:563-575)."""
+    c = Circuit(num_qubits).with_name("XYZ-Heisenberg Model")
+    for _ in range(params.time_steps):
+        for i in range(num_qubits - 1):
+            c.rxx(i, i + 1, 2.0 * (params.jx * params.dt))
+        for i in range(num_qubits - 1):
+            c.ryy(i, i + 1, 2.0 * (params.jy * params.dt))
+        for i in range(num_qubits - 1):
+            c.rzz(i, i + 1, 2.0 * (params.jz * params.dt))
+        if params.external_field != 0.0:
+            for i in range(num_qubits):
+                c.rz(i, params.external_field * params.dt)
+    return c
+
+
+def xyz_heisenberg_simulate(state, params):
+    """xyz_heisenberg::simulate (xyz_heisenberg.rs:64-67)."""
+    xyz_heisenberg_circuit(state.num_qubits(), params).execute(state)
+
+
+def xyz_heisenberg_observable(num_qubits, params):
+    """The open-chain Hamiltonian of calculate_energy_efficient
+    (xyz_heisenberg.rs:143-162): +Jx XX + Jy YY + Jz ZZ on bonds (i, i+1),
+    then h Z_i when h != 0, as a Pauli sum.  YY carries two Y factors, so
+    the reference's Y phase convention (expectation_yy, 190-227) gives the
+    physical value."""
+    from .optimization import Pauli, PauliObservable, PauliTerm
+    o = PauliObservable(num_qubits)
+    for i in range(num_qubits - 1):
+        for pa, c in ((Pauli.X, params.jx), (Pauli.Y, params.jy), (Pauli.Z, params.jz)):
+            p = [Pauli.I] * num_qubits
+            p[i] = p[i + 1] = pa
+            o.add_term(PauliTerm(c, p))
+    if params.external_field != 0.0:
+        for i in range(num_qubits):
+            p = [Pauli.I] * num_qubits
+            p[i] = Pauli.Z
+            o.add_term(PauliTerm(params.external_field, p))
+    return o
+
+
+def xyz_energy_efficient(state, params):
+    """calculate_energy_efficient (xyz_heisenberg.rs:143-162) on the device:
+    one Pauli-sum expectation with f64 accumulation."""
+    return xyz_heisenberg_observable(state.num_qubits(), params).expectation(state)
 
 
 def brickwork_circuit(num_qubits, depth, seed):

@@ -103,6 +103,13 @@ class Circuit:
         self._n = int(num_qubits)
         self._ops = []
         self._name = None
+        self._noise = []
+
+    def add_noise(self, noise_model):
+        """Circuit::add_noise (mod.rs:339-342): applied after every
+        operation by execute."""
+        self._noise.append(noise_model)
+        return self
 
     def with_name(self, name):
         self._name = name
@@ -298,10 +305,29 @@ class Circuit:
         return [op.gate if isinstance(op.gate, abi.LqGate) else None for op in self._ops]
 
     def execute(self, state, mode=None):
+        """Circuit::execute (mod.rs:352-375): every operation in order, then
+        every noise model after each operation when there are any."""
+        if not self._noise:
+            return self.execute_without_noise(state, mode)
+        self._check_state(state)
+        for op in self._ops:
+            if isinstance(op.gate, abi.LqGate):
+                call("lq_apply_gate", state.handle, ctypes.byref(op.gate))
+            else:
+                op.gate.apply(state)
+            for model in self._noise:
+                model.apply(state)
+
+    def _check_state(self, state):
         if state.num_qubits() != self._n:
             raise CircuitQubitMismatch(abi.LQ_ERR_CIRCUIT_QUBIT_MISMATCH,
                                        "Circuit qubit count mismatch: circuit has %d, state has %d"
                                        % (self._n, state.num_qubits()))
+
+    def execute_without_noise(self, state, mode=None):
+        """Circuit::execute_without_noise (mod.rs:196-209); also execute
+        when no noise model is attached."""
+        self._check_state(state)
         if mode is None:
             mode = default_exec_mode()
         run = []
@@ -313,8 +339,6 @@ class Circuit:
             run = []
             op.gate.apply(state)
         self._flush(state, run, mode)
-
-    execute_without_noise = execute
 
     def execute_and_measure(self, uniforms=None):
         """Circuit::execute_and_measure (mod.rs:215-233, the default parallel

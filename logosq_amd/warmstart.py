@@ -402,6 +402,34 @@ def cmd_evaluate(a):
         np.median([v for v in iw if v is not None]) if any(v is not None for v in iw) else None), flush=True)
 
 
+def cmd_merge(a):
+    """Join evaluation parts (disjoint --offset ranges of the held-out
+    couplings, same training recipe and learning rates) into one file."""
+    parts = []
+    for path in a.parts:
+        with open(path) as f:
+            parts.append(json.load(f))
+    keys = ("lr_plain", "lr_warm", "epochs", "lr", "max_iterations", "tolerance")
+    for p in parts[1:]:
+        for k in keys:
+            if p[k] != parts[0][k]:
+                raise SystemExit("parts differ in %s: %r vs %r" % (k, p[k], parts[0][k]))
+    parts.sort(key=lambda p: p["offset"])
+    rows, end = [], 0
+    for p in parts:
+        if p["offset"] != end:
+            raise SystemExit("parts do not tile the held-out couplings: offset %d after %d" % (p["offset"], end))
+        rows += p["rows"]
+        end += len(p["rows"])
+    out = dict(parts[0])
+    out["rows"] = rows
+    out["parts"] = [{"offset": p["offset"], "count": len(p["rows"]),
+                     "validation_median_iterations": p["validation_median_iterations"]} for p in parts]
+    with open(a.out, "w") as f:
+        json.dump(out, f)
+    print("merged %d parts, %d couplings -> %s" % (len(parts), len(rows), a.out))
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="python -m logosq_amd.warmstart")
@@ -424,8 +452,11 @@ def main(argv=None):
     e.add_argument("--lr-plain", type=float, default=None)
     e.add_argument("--lr-warm", type=float, default=None)
     e.add_argument("--lr-grid", default="0.02,0.05,0.1")
+    m = sub.add_parser("merge", help="join evaluation parts into one file")
+    m.add_argument("--out", required=True)
+    m.add_argument("parts", nargs="+")
     a = ap.parse_args(argv)
-    {"generate": cmd_generate, "evaluate": cmd_evaluate}[a.cmd](a)
+    {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge}[a.cmd](a)
 
 
 if __name__ == "__main__":
