@@ -282,6 +282,9 @@ def warmstart_bench(lq, count=4):
             ts.append(time.perf_counter() - t)
         return min(ts)
     t_seq = wall(lambda: seq.compute_gradient(ans, obs, theta0, energy=True), 1)
+    t_seqf = wall(lambda: sequential_fused_ps(lq, ans, obs, theta0), 1)
+    g_sf = sequential_fused_ps(lq, ans, obs, theta0)[1]
+    g_bat = np.array(bat.compute_gradient(ans, obs, theta0))
     t_bat = wall(lambda: bat.compute_gradient(ans, obs, theta0, energy=True), 2)
     t_adj = wall(lambda: adj.compute_gradient(ans, obs, theta0, energy=True), 3)
     t_start = wall(lambda: warmstart.choose_starts(ws, ans, Jte[:1]), 2)
@@ -291,9 +294,9 @@ def warmstart_bench(lq, count=4):
     r_adj = lq.VQE(ans, obs, adj, lq.Adam(lr_w, 3)).run(start)
     traj = float(np.max(np.abs(np.array(r_ps.convergence_history) - r_adj.convergence_history)))
 
-    def times(rs, kp, kw):
-        tp = [(r[kp] or cap) * t_seq for r in rs]
-        tw = [t_start + (r[kw] or cap) * t_bat for r in rs]
+    def times(rs, kp, kw, t_plain=t_seq, t_warm=t_bat):
+        tp = [(r[kp] or cap) * t_plain for r in rs]
+        tw = [t_start + (r[kw] or cap) * t_warm for r in rs]
         return tp, tw
 
     def summary(rs):
@@ -311,6 +314,12 @@ def warmstart_bench(lq, count=4):
                          "median_time_to_dE_s_warm_batched": round(float(np.median(tw)), 2),
                          "speedup_median": round(float(np.median(tp) / np.median(tw)), 2),
                          "speedup_total": round(float(np.sum(tp) / np.sum(tw)), 2)}
+            # the same iterations against the strongest one-circuit-at-a-time
+            # plain method (fused execution), and with the adjoint gradient
+            tpf, _ = times(rs, kp, kw, t_plain=t_seqf)
+            _, twa = times(rs, kp, kw, t_warm=t_adj)
+            out[name]["speedup_median_vs_plain_fused_sequential"] = round(float(np.median(tpf) / np.median(tw)), 2)
+            out[name]["speedup_median_warm_adjoint_vs_plain"] = round(float(np.median(tp) / np.median(twa)), 1)
         out["warm_final_energy_lower"] = sum(r["e_warm_final"] < r["e_plain_final"] - 1e-4 for r in rs)
         out["proposal_used"] = sum(r["used_proposal"] for r in rs)
         return out
@@ -319,9 +328,12 @@ def warmstart_bench(lq, count=4):
                        "live, iteration cap %d" % (len(rows), cap),
            "train_s": round(t_train, 2), "train_loss_first_last": [float(hist[0]), float(hist[-1])],
            "lr_plain": lr_p, "lr_warm": lr_w,
-           "s_per_iteration": {"plain_sequential_ps": round(t_seq, 4), "batched_ps": round(t_bat, 4),
+           "s_per_iteration": {"plain_sequential_ps": round(t_seq, 4),
+                               "plain_sequential_ps_fused_execution": round(t_seqf, 4),
+                               "batched_ps": round(t_bat, 4),
                                "adjoint": round(t_adj, 5), "warm_start_proposal": round(t_start, 4)},
            "trajectory_batched_ps_vs_adjoint_max_abs": traj,
+           "gradient_fused_sequential_vs_batched_max_abs": float(np.max(np.abs(g_sf - g_bat))),
            "live": summary(rows)}
     if ev:
         # the live couplings are the first held-out ones: same iterations
@@ -334,6 +346,44 @@ def warmstart_bench(lq, count=4):
         out["held_out_%d" % len(ev["rows"])]["source"] = ("logosq_amd/data/warmstart_xyz20_L8_eval.json (iterations), "
                                                         "per-iteration times measured here")
     return out
+
+
+def sequential_fused_ps(lq, ans, obs, theta, shift=math.pi / 2):
+    """One parameter-shift gradient with the 2P+1 circuits run one after
+    another, each with fused tile-pass execution (lq_execute FUSED) and one
+    expectation: the strongest one-circuit-at-a-time GPU method.  The bound
+    gate array is built once; only the shifted angle changes per circuit."""
+    import numpy as np
+    from logosq_amd import abi, algorithms
+    tmpl = ans.template()
+    n = ans.num_qubits()
+    arr = abi.gate_array(algorithms.bind(tmpl, theta))
+    slot = {}
+    for i, g in enumerate(tmpl):
+        for k in range(3):
+            if g.param[k] >= 0:
+                slot[g.param[k]] = (i, k)
+    codes, coeffs = obs.codes()
+    st = lq.State(n)
+    e = ctypes.c_double()
+    cp = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    fp = coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def energy():
+        lq._lib.call("lq_state_set_basis", st.handle, 0)
+        lq._lib.call("lq_execute", st.handle, arr, len(tmpl), abi.EXEC_FUSED)
+        lq._lib.call("lq_expect_pauli_sum", st.handle, cp, fp, len(coeffs), obs.convention, ctypes.byref(e))
+        return e.value
+    e0 = energy()
+    grad = np.zeros(len(theta))
+    for p, (i, k) in slot.items():
+        arr[i].theta[k] = theta[p] + shift
+        ep = energy()
+        arr[i].theta[k] = theta[p] - shift
+        em = energy()
+        arr[i].theta[k] = theta[p]
+        grad[p] = (ep - em) / 2.0
+    return e0, grad
 
 
 def cpu_baseline(nq, ngates, seed, depth):
@@ -379,6 +429,27 @@ def run_circuit(lq, st, arr, count, mode, steps, warmup, label):
         times.append(time.perf_counter() - t0)
         print("bench: %s step %.3f s" % (label, times[-1]), file=sys.stderr, flush=True)
     return times
+
+
+def c64_error(lq, n, depth, seed):
+    """The brickwork circuit at n qubits in complex64 (per-gate and fused)
+    against complex128: max |amplitude difference| and the f64-accumulated
+    norm of the complex64 result."""
+    import numpy as np
+    from logosq_amd import abi
+    descs = lq.algorithms.brickwork_circuit(n, depth, seed).descriptors()
+    arr = abi.gate_array(descs)
+    s128 = lq.State(n, lq.C128)
+    lq._lib.call("lq_execute", s128.handle, arr, len(descs), abi.EXEC_PER_GATE)
+    ref = s128.vector()
+    s128 = None
+    out = {"qubits": n}
+    for name, mode in (("per_gate", abi.EXEC_PER_GATE), ("fused", abi.EXEC_FUSED)):
+        s = lq.State(n, lq.C64)
+        lq._lib.call("lq_execute", s.handle, arr, len(descs), mode)
+        out[name + "_max_abs_err"] = float(np.max(np.abs(s.vector() - ref)))
+        out[name + "_norm_sqr"] = s.norm_sqr()
+    return out
 
 
 def event_time(lq, fn):
@@ -472,6 +543,7 @@ def main():
         c64 = {"rx_pair_GBps": round(2 * 8 * N * len(rx) / t64 / 1e9, 1),
                "cnot_GBps": round(8 * N * len(cn) / t64c / 1e9, 1)}
         s64 = None
+        c64["error_vs_c128"] = c64_error(lq, min(a.qubits, 24), a.depth, a.seed)
     extra = {}
     if not a.no_extra:
         print("bench: gradient workload", file=sys.stderr, flush=True)

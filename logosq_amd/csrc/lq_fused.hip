@@ -809,21 +809,8 @@ __device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t
     }
 }
 
-// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS
-// accesses (lgkmcnt), not for its global loads, so a tile prefetched into
-// registers stays in flight across the group passes.  The "memory" clobber
-// keeps the compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
 // At least two waves per SIMD: two 64 KiB tiles per CU are resident, so the
 // register budget is 256 VGPRs per lane.
-//
-// The swizzle is linear over XOR, so a group's 16 subcube offsets are
-// swizzled once per group (uniform values) and each element address is one
-// XOR.  Complex64 tiles are software-pipelined: the block's next tile is
-// loaded into registers right after the current one is written to LDS and
-// lands while its groups run; LDS hand-offs use lds_barrier so those loads
-// are not waited for early.
 template <int K, bool RICH, typename V, typename T>
 __global__ void __launch_bounds__(tile_threads(K), 2) k_tileg(V* __restrict__ psi, TilePass p,
                                                           const TileOp* __restrict__ ops,
@@ -836,10 +823,6 @@ __global__ void __launch_bounds__(tile_threads(K), 2) k_tileg(V* __restrict__ ps
     constexpr uint32_t NSUB = TS >> 4;
     constexpr bool SFULL = NSUB >= (uint32_t)NT;
     constexpr int SPT = SFULL ? (int)(NSUB / NT) : 1;
-    // Prefetch the next tile into registers only where it fits the 256-VGPR
-    // budget without spills (complex64); complex128 needs 64 more VGPRs than
-    // the group code leaves.
-    constexpr bool PF = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     V* tile = reinterpret_cast<V*>(smem);
     uint64_t* hoff = reinterpret_cast<uint64_t*>(smem + TS * sizeof(V));
@@ -855,91 +838,63 @@ __global__ void __launch_bounds__(tile_threads(K), 2) k_tileg(V* __restrict__ ps
     __syncthreads();
     const TileGroup* grp = groups + p.group_first;
     const uint64_t total = ninst * p.ntiles;
-    // tile w -> (state base pointer, amplitude base index); wave-uniform by
-    // construction, readfirstlane lets the compiler see it
-    auto locate = [&](uint64_t w, V*& s, const TileOp*& op, uint64_t& base) {
+    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        // wave-uniform by construction; readfirstlane lets the compiler see it,
+        // so op parameters are fetched with scalar loads
         const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));
         const uint64_t t = w - (uint64_t)inst * p.ntiles;
-        s = psi + (uint64_t)inst * inst_stride;
-        op = ops + (uint64_t)inst * ops_stride + p.op_first;
-        base = t << L;
+        V* s = psi + (uint64_t)inst * inst_stride;
+        const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;
+        uint64_t base = t << L;
         for (int j = 0; j < p.hb; ++j) base = insert_zero(base, p.hpos[j]);
-    };
-    auto gidx = [&](uint64_t base, uint32_t e) -> uint64_t { return base | (e & lowmask) | hoff[e >> L]; };
-    // Native two-element vectors: arrays of HIP's vector structs are not
-    // always split into registers.
-    typedef T N2 __attribute__((ext_vector_type(2)));
-    N2 r[EPT];
-    V* s = nullptr;
-    const TileOp* op = nullptr;
-    uint64_t base = 0;
-    auto load_tile = [&](V* src, uint64_t b) {
+        auto gidx = [&](uint32_t e) -> uint64_t { return base | (e & lowmask) | hoff[e >> L]; };
+        {
+            V r[EPT];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) r[j] = *reinterpret_cast<const N2*>(src + gidx(b, tid + (uint32_t)j * NT));
-    };
-    if (PF && blockIdx.x < total) {
-        locate(blockIdx.x, s, op, base);
-        load_tile(s, base);
-    }
-    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
-        if constexpr (!PF) {
-            locate(w, s, op, base);
-            load_tile(s, base);
-        }
+            for (int j = 0; j < EPT; ++j) r[j] = s[gidx(tid + (uint32_t)j * NT)];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) *reinterpret_cast<N2*>(tile + swz(tid + (uint32_t)j * NT)) = r[j];
-        lds_barrier();
-        V* sn = nullptr;
-        const TileOp* opn = nullptr;
-        uint64_t basen = 0;
-        if constexpr (PF) {
-            // the block's next tile, in flight while this one's groups run
-            if (w + gridDim.x < total) {
-                locate(w + gridDim.x, sn, opn, basen);
-                load_tile(sn, basen);
-            }
+            for (int j = 0; j < EPT; ++j) tile[swz(tid + (uint32_t)j * NT)] = r[j];
         }
+        __syncthreads();
         for (int gi = 0; gi < p.group_count; ++gi) {
             const TileGroup& G = grp[gi];
             const int s0 = G.spos[0], s1 = G.spos[1], s2 = G.spos[2], s3 = G.spos[3];
-            // swizzled offset of subcube slot c (uniform across the group)
-            auto dsw = [&](int c) -> uint32_t {
-                return swz(((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
-                           ((uint32_t)((c >> 2) & 1) << s2) | ((uint32_t)((c >> 3) & 1) << s3));
-            };
 #pragma unroll
             for (int k = 0; k < SPT; ++k) {
                 const uint32_t q = tid + (uint32_t)k * NT;
                 if (!SFULL && q >= NSUB) break;
                 const uint32_t eb = ins0(ins0(ins0(ins0(q, s0), s1), s2), s3);
-                const uint32_t ebs = swz(eb);
-                const uint64_t gb = gidx(base, eb);
+                const uint64_t gb = gidx(eb);
                 T xr[16], xi[16];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
-                    const V v = tile[ebs ^ dsw(c)];
+                    const uint32_t e = eb | ((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
+                                       ((uint32_t)((c >> 2) & 1) << s2) | ((uint32_t)((c >> 3) & 1) << s3);
+                    const V v = tile[swz(e)];
                     xr[c] = v.x;
                     xi[c] = v.y;
                 }
                 for (int o = G.op_first; o < G.op_first + G.op_count; ++o) g_apply<RICH, T>(xr, xi, op[o], gb);
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
+                    const uint32_t e = eb | ((uint32_t)(c & 1) << s0) | ((uint32_t)((c >> 1) & 1) << s1) |
+                                       ((uint32_t)((c >> 2) & 1) << s2) | ((uint32_t)((c >> 3) & 1) << s3);
                     V v;
                     v.x = xr[c];
                     v.y = xi[c];
-                    tile[ebs ^ dsw(c)] = v;
+                    tile[swz(e)] = v;
                 }
             }
-            lds_barrier();
+            __syncthreads();
         }
+        {
+            V r[EPT];
 #pragma unroll
-        for (int j = 0; j < EPT; ++j) s[gidx(base, tid + (uint32_t)j * NT)] = tile[swz(tid + (uint32_t)j * NT)];
-        lds_barrier();
-        if constexpr (PF) {
-            s = sn;
-            op = opn;
-            base = basen;
+            for (int j = 0; j < EPT; ++j) r[j] = tile[swz(tid + (uint32_t)j * NT)];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) s[gidx(tid + (uint32_t)j * NT)] = r[j];
         }
+        __syncthreads();
     }
 }
 

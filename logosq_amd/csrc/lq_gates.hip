@@ -14,7 +14,8 @@
 //     both halves of its pair and updates them in registers;
 //   - pair target among them: the partner lives in lane (lane ^ 2^tl) of the
 //     same wavefront and is fetched with a cross-lane shuffle.
-// SWAP and 4x4 blocks enumerate both of their bits.
+// SWAP and 4x4 blocks enumerate their bits above the six lowest; 4x4 bits
+// below bit 6 stay lane bits and are exchanged by shuffles.
 #include <cmath>
 #include <cstring>
 
@@ -187,10 +188,16 @@ __global__ void __launch_bounds__(kThreads) k_pair_lo(V* __restrict__ psi, uint6
     for (int j = 0; j < U; ++j) {
         V p = shfl_xor_c(a[j], pmask);  // every lane takes part
         if (idx[j] != ~0ull && live) {
-            V o0, o1;
+            V o0, o1, r = a[j];
             if (hi_half) pair_op<OP, V, T>(m, p, a[j], o0, o1);
             else pair_op<OP, V, T>(m, a[j], p, o0, o1);
-            gstore<NTM>(psi + idx[j], !on ? a[j] : hi_half ? o1 : o0);
+            // component-wise selects: a select of whole vector structs is
+            // lowered through scratch memory
+            if (on) {
+                r.x = hi_half ? o1.x : o0.x;
+                r.y = hi_half ? o1.y : o0.y;
+            }
+            gstore<NTM>(psi + idx[j], r);
         }
     }
 }
@@ -224,7 +231,9 @@ __global__ void __launch_bounds__(kThreads) k_diag(V* __restrict__ psi, uint64_t
             if (sel1 >= 0) k |= (int)((i >> sel1) & 1ull) << 1;
             const T pr = k == 3 ? d.d[6] : k == 2 ? d.d[4] : k == 1 ? d.d[2] : d.d[0];
             const T pi = k == 3 ? d.d[7] : k == 2 ? d.d[5] : k == 1 ? d.d[3] : d.d[1];
-            gstore<NTM>(psi + i, on ? cmul(pr, pi, a[j]) : a[j]);
+            V r = a[j];
+            if (on) r = cmul(pr, pi, a[j]);
+            gstore<NTM>(psi + i, r);
         }
     }
 }
@@ -289,6 +298,108 @@ __global__ void __launch_bounds__(kThreads) k_quad(V* __restrict__ psi, uint64_t
 #pragma unroll
                 for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m.m[8 * r + 2 * c], m.m[8 * r + 2 * c + 1], a[j][c]));
                 gstore<NTM>(psi + (idx[j] | off[r]), acc);
+            }
+        }
+    }
+}
+
+// 4x4 block with one (NLO = 1) or both (NLO = 2) of its bits among the six
+// lowest bits.  Those are lane bits (la for bit a, lb for bit b; -1 for an
+// enumerated bit, whose mask is mh), so every load and store of a wavefront
+// is one coalesced run; a lane's partners come through cross-lane shuffles
+// and the lane computes its own outputs with its row of the block (selected
+// per lane by its own bit values).  Sub-index (bit a << 1) | bit b.
+template <typename T>
+__device__ __forceinline__ T pick2(bool c, T x, T y) { return c ? y : x; }
+
+template <typename V, typename T, int NLO, int U, bool NTM>
+__global__ void __launch_bounds__(kThreads) k_quad_lo(V* __restrict__ psi, uint64_t units, SkipBits sk,
+                                                      uint32_t lane_limit, int la, int lb, uint64_t mh,
+                                                      Mat4<T> m) {
+    LQ_WAVE_IDS
+    const uint64_t loff = spread(lane, sk) | sk.ones;
+    const bool live = lane < lane_limit;
+    constexpr int NV = NLO == 2 ? 1 : 2;  // amplitudes per lane
+    V v[U][NV];
+    uint64_t idx[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const uint64_t u = LQ_UNIT(j);
+        idx[j] = ~0ull;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            v[j][k].x = 0;
+            v[j][k].y = 0;
+        }
+        if (u < units) {
+            idx[j] = spread(u << kLaneBits, sk) | loff;
+            if (live) {
+                v[j][0] = gload<NTM>(psi + idx[j]);
+                if constexpr (NV == 2) v[j][1] = gload<NTM>(psi + (idx[j] | mh));
+            }
+        }
+    }
+    if constexpr (NLO == 2) {
+        const int xa = (lane >> la) & 1, xb = (lane >> lb) & 1;
+        const int so = (xa << 1) | xb;  // own sub-index
+        // w_e = amplitude with sub-index so ^ e; its coefficient M[so][so ^ e]
+        T cr[4], ci[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            T r4[4], i4[4];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                r4[s4] = m.m[8 * s4 + 2 * (s4 ^ e)];
+                i4[s4] = m.m[8 * s4 + 2 * (s4 ^ e) + 1];
+            }
+            cr[e] = pick2(xa, pick2(xb, r4[0], r4[1]), pick2(xb, r4[2], r4[3]));
+            ci[e] = pick2(xa, pick2(xb, i4[0], i4[1]), pick2(xb, i4[2], i4[3]));
+        }
+        const int mask[4] = {0, 1 << lb, 1 << la, (1 << la) | (1 << lb)};
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            V w[4];
+            w[0] = v[j][0];
+#pragma unroll
+            for (int e = 1; e < 4; ++e) w[e] = shfl_xor_c(v[j][0], mask[e]);  // every lane takes part
+            if (idx[j] != ~0ull && live) {
+                V acc = cmul(cr[0], ci[0], w[0]);
+#pragma unroll
+                for (int e = 1; e < 4; ++e) acc = cadd(acc, cmul(cr[e], ci[e], w[e]));
+                gstore<NTM>(psi + idx[j], acc);
+            }
+        }
+    } else {
+        const bool lo_is_a = la >= 0;
+        const int lbit = lo_is_a ? la : lb;
+        const int x = (lane >> lbit) & 1;  // own value of the lane bit
+        // sub-index of (lane-bit value xl, enumerated-bit value yh)
+        auto sub = [&](int xl, int yh) { return lo_is_a ? ((xl << 1) | yh) : ((yh << 1) | xl); };
+        // coefficient of (same / other lane value, yh) in output row (x, yo)
+        T cr[2][2][2], ci[2][2][2];
+#pragma unroll
+        for (int yo = 0; yo < 2; ++yo)
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int yh = 0; yh < 2; ++yh) {
+                    const int r0 = sub(0, yo), c0 = sub(o, yh);          // own lane value 0
+                    const int r1 = sub(1, yo), c1 = sub(1 - o, yh);      // own lane value 1
+                    cr[yo][o][yh] = pick2(x, m.m[8 * r0 + 2 * c0], m.m[8 * r1 + 2 * c1]);
+                    ci[yo][o][yh] = pick2(x, m.m[8 * r0 + 2 * c0 + 1], m.m[8 * r1 + 2 * c1 + 1]);
+                }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const V p0 = shfl_xor_c(v[j][0], 1 << lbit), p1 = shfl_xor_c(v[j][1], 1 << lbit);
+            if (idx[j] != ~0ull && live) {
+#pragma unroll
+                for (int yo = 0; yo < 2; ++yo) {
+                    V acc = cmul(cr[yo][0][0], ci[yo][0][0], v[j][0]);
+                    acc = cadd(acc, cmul(cr[yo][0][1], ci[yo][0][1], v[j][1]));
+                    acc = cadd(acc, cmul(cr[yo][1][0], ci[yo][1][0], p0));
+                    acc = cadd(acc, cmul(cr[yo][1][1], ci[yo][1][1], p1));
+                    gstore<NTM>(psi + (yo ? (idx[j] | mh) : idx[j]), acc);
+                }
             }
         }
     }
@@ -517,20 +628,32 @@ template <typename V, typename T>
 static void quad_t(lq_state* st, int ba, int bb, const double* md) {
     Mat4<T> m;
     for (int i = 0; i < 32; ++i) m.m[i] = (T)md[i];
-    const Geometry g = geometry(st->n, 0, (1ull << ba) | (1ull << bb), st->amp_bytes);
     V* psi = (V*)st->d_amps;
-    // The 4x4 kernel reads all four amplitudes of a lane's group; with a bit
-    // below the wave's span its four loads share cache lines, which the
-    // cached (temporal) form fetches once.
-    with_variant([&]<int U, bool NT>() {
-        const unsigned blocks = blocks_for_units(g.units, U);
-        if (NT && ba >= kLaneBits && bb >= kLaneBits)
-            hipLaunchKernelGGL((k_quad<V, T, U, true>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
-                               g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
-        else
-            hipLaunchKernelGGL((k_quad<V, T, U, false>), dim3(blocks), dim3(kThreads), 0, stream(), psi, g.units,
-                               g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
-    });
+    const bool lo_a = ba < kLaneBits, lo_b = bb < kLaneBits;
+    if (!lo_a && !lo_b) {
+        // both bits enumerated: four coalesced loads per lane
+        const Geometry g = geometry(st->n, 0, (1ull << ba) | (1ull << bb), st->amp_bytes);
+        with_variant([&]<int U, bool NT>() {
+            hipLaunchKernelGGL((k_quad<V, T, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
+                               stream(), psi, g.units, g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
+        });
+    } else if (lo_a && lo_b) {
+        // both among the lane bits (nothing skipped, so lane bit = amplitude bit)
+        const Geometry g = geometry(st->n, 0, 0, st->amp_bytes);
+        with_variant([&]<int U, bool NT>() {
+            hipLaunchKernelGGL((k_quad_lo<V, T, 2, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
+                               stream(), psi, g.units, g.sk, g.lane_limit, ba, bb, (uint64_t)0, m);
+        });
+    } else {
+        // one lane bit (below bit 6, so below the skipped one), one enumerated
+        const int hb = lo_a ? bb : ba;
+        const Geometry g = geometry(st->n, 0, 1ull << hb, st->amp_bytes);
+        with_variant([&]<int U, bool NT>() {
+            hipLaunchKernelGGL((k_quad_lo<V, T, 1, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
+                               stream(), psi, g.units, g.sk, g.lane_limit, lo_a ? ba : -1, lo_b ? bb : -1,
+                               1ull << hb, m);
+        });
+    }
 }
 
 int launch_quad(lq_state* st, int ba, int bb, const double m[32]) {
