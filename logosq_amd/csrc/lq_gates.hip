@@ -61,11 +61,12 @@ __device__ __forceinline__ void gstore(V* p, V a) {
 
 // Bit positions that a launch's lanes and work units skip, ascending: the
 // fixed bits (value 1 in every amplitude the gate touches: controls, the
-// 1-half of phase gates) and the bits whose values a lane enumerates itself
-// (a pair kernel's high target, both bits of a SWAP or 4x4 block).  The
-// remaining "free" bits are numbered from the bottom: lane l covers the 6
-// lowest free bits and the work-unit index the rest, so a wavefront's 64
-// amplitudes are the densest run of amplitudes the gate must touch.
+// 1-half of phase gates) except those with runs shorter than kDenseBytes,
+// and the bits whose values a lane enumerates itself (a pair kernel's high
+// target, a SWAP's bits, a 4x4 block's bits above bit 5).  The remaining
+// "free" bits are numbered from the bottom: lane l covers the 6 lowest free
+// bits and the work-unit index the rest, so a wavefront's 64 amplitudes are
+// the densest run of amplitudes the gate must touch.
 constexpr int kMaxSkip = 5;
 struct SkipBits {
     // seg[i]: the bits of a compressed free-bit index that move up by i
