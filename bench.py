@@ -431,6 +431,27 @@ def run_circuit(lq, st, arr, count, mode, steps, warmup, label):
     return times
 
 
+def expectation_bench(lq, st, n, c, reps=3):
+    """Pauli-sum expectation on the circuit's output state: the periodic XYZ
+    ring, 3n terms with J = (0.7, -0.4, 0.9).  The XX and YY terms of a bond
+    share one flip mask and every ZZ term is diagonal, so the sum is n + 1
+    groups, each one read of the state (c*N bytes).  HIP-event time of the
+    whole lq_expect_pauli_sum call (fastest of `reps`)."""
+    import numpy as np
+    codes, coeffs = lq.algorithms.heisenberg_codes(n, 0.7, -0.4, 0.9)
+    groups = len({tuple(((row == 1) | (row == 2)).tolist()) for row in codes})
+    e = ctypes.c_double()
+    cp = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    fp = coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    fn = lambda: lq._lib.call("lq_expect_pauli_sum", st.handle, cp, fp, len(coeffs), 0, ctypes.byref(e))
+    fn()
+    t = min(event_time(lq, fn) for _ in range(reps))
+    gbs = groups * c * (1 << n) / t / 1e9
+    return {"workload": "periodic XYZ ring, %d terms, %d flip groups" % (len(coeffs), groups),
+            "ms": round(t * 1e3, 3), "bytes": groups * c * (1 << n), "GBps": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "energy": e.value}
+
+
 def c64_error(lq, n, depth, seed):
     """The brickwork circuit at n qubits in complex64 (per-gate and fused)
     against complex128: max |amplitude difference| and the f64-accumulated
@@ -491,6 +512,8 @@ def main():
           file=sys.stderr, flush=True)
     times = run_circuit(lq, st, arr, len(descs), mode, a.steps, a.warmup, a.mode)
     norm2 = st.norm_sqr()
+    expect = expectation_bench(lq, st, a.qubits, c)
+    print("bench: expectation %.1f GB/s" % expect["GBps"], file=sys.stderr, flush=True)
     step_s = sum(times) / len(times)
     bytes_gates = algorithmic_bytes(descs, a.qubits, c)
     passes = num_passes(lq, a.qubits, dtype, descs)
@@ -577,7 +600,8 @@ def main():
                    "per_gate_kernels": {"rx_pair_GBps": round(rx_gbs, 1), "rx_frac": round(rx_gbs / HBM_PEAK_GBS, 4),
                                         "cnot_GBps": round(cn_gbs, 1), "cnot_frac": round(cn_gbs / HBM_PEAK_GBS, 4),
                                         "c64": c64, "by_kind": by_kind},
-                   "fused_execution": fused},
+                   "fused_execution": fused,
+                   "expectation": expect},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.mode),
                      "kernel": "%s, %d B per launch, avg %.3f ms" % (kname, bpl, avg_s * 1e3)},

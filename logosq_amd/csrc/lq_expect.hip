@@ -34,55 +34,122 @@ __device__ __forceinline__ double block_sum_e(double v, double* sh) {
     return sh[0];
 }
 
-// x == 0: diagonal group, sum_i |psi_i|^2 * sum_t cr_t (-1)^{popc(i & zy_t)}.
+// x == 0: diagonal group, sum_i |psi_i|^2 * w(i), w(i) = sum_t cr_t
+// (-1)^{popc(i & zy_t)}.  A wavefront reads 64 consecutive amplitudes (one
+// chunk, i = chunk * 64 + lane) and walks R consecutive chunks per work item
+// (R = 64, fewer for small states), so bits 6..6+log2(R) of i are the step r
+// and the rest are fixed for the item.  Terms whose mask misses the r bits
+// ("static": for a local Hamiltonian, most of them) are summed once per item
+// and lane into w_static; the others ("dynamic", at most kDiagDyn per launch)
+// keep their fixed-bit sign in d_t, and their r-sign comes from a 64-bit
+// table per term (bit r = parity of r & the term's r bits) on the scalar
+// unit, so a chunk costs one FMA per dynamic term.  The host orders each
+// launch's terms static first.
+constexpr int kDiagDyn = 16;
 template <typename V>
-__global__ void __launch_bounds__(kExpThreads) k_expect_diag(const V* __restrict__ psi, uint64_t dim,
-                                                             const PauliTermDev* __restrict__ terms, int nt,
+__global__ void __launch_bounds__(kExpThreads) k_expect_diag(const V* __restrict__ psi, uint64_t items, uint32_t R,
+                                                             uint32_t lane_limit,
+                                                             const PauliTermDev* __restrict__ terms, int ns, int nd,
                                                              double* __restrict__ partial) {
     __shared__ double sh[kExpThreads];
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    double acc = 0.0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
-        V a = psi[i];
-        double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-        double w = 0.0;
-        for (int t = 0; t < nt; ++t) {
-            double c = terms[t].cr;
-            w += (__popcll(i & terms[t].zy) & 1) ? -c : c;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t w0 = (uint64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * (kExpThreads / 64) +
+                                                                 (threadIdx.x >> 6));
+    const uint64_t W = (uint64_t)gridDim.x * (kExpThreads / 64);
+    // r-sign tables of the dynamic terms (uniform)
+    uint64_t rsign[kDiagDyn];
+#pragma unroll
+    for (int t = 0; t < kDiagDyn; ++t) {
+        uint64_t m = 0;
+        if (t < nd) {
+            const uint64_t zr = (terms[ns + t].zy >> 6) & (uint64_t)(R - 1);
+            for (uint32_t r = 0; r < R; ++r) m |= (uint64_t)(__popcll(r & zr) & 1) << r;
         }
-        acc += p * w;
+        rsign[t] = m;
+    }
+    constexpr uint32_t U = 4;  // chunks in flight per wave
+    double acc = 0.0;
+    if (lane < lane_limit) {
+        for (uint64_t it = w0; it < items; it += W) {
+            const uint64_t base = ((it * R) << 6) | lane;
+            double wst = 0.0;
+            for (int t = 0; t < ns; ++t) wst += (__popcll(base & terms[t].zy) & 1) ? -terms[t].cr : terms[t].cr;
+            double d[kDiagDyn];
+#pragma unroll
+            for (int t = 0; t < kDiagDyn; ++t) {
+                d[t] = 0.0;
+                if (t < nd) d[t] = (__popcll(base & terms[ns + t].zy) & 1) ? -terms[ns + t].cr : terms[ns + t].cr;
+            }
+            for (uint32_t r = 0; r < R; r += U) {
+                V a[U];
+#pragma unroll
+                for (uint32_t j = 0; j < U; ++j)
+                    if (r + j < R) a[j] = psi[base | ((uint64_t)(r + j) << 6)];
+#pragma unroll
+                for (uint32_t j = 0; j < U; ++j) {
+                    if (r + j >= R) break;
+                    const double p = (double)a[j].x * (double)a[j].x + (double)a[j].y * (double)a[j].y;
+                    double w = wst;
+#pragma unroll
+                    for (int t = 0; t < kDiagDyn; ++t)
+                        if (t < nd) w = __builtin_fma(((rsign[t] >> (r + j)) & 1) ? -1.0 : 1.0, d[t], w);
+                    acc += p * w;
+                }
+            }
+        }
     }
     double s = block_sum_e<kExpThreads>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x != 0: pairs {i, j = i ^ x} with bit h (top bit of x) of i clear.
+// x != 0: pairs {i, j = i ^ x} with bit h (top bit of x) of i clear.  A
+// wavefront takes 64 consecutive pair indices k (i = k with a zero inserted
+// at bit h), U chunks of them in flight.
 template <typename V>
-__global__ void __launch_bounds__(kExpThreads) k_expect_flip(const V* __restrict__ psi, uint64_t half, uint64_t x,
-                                                             int h, const PauliTermDev* __restrict__ terms, int nt,
+__global__ void __launch_bounds__(kExpThreads) k_expect_flip(const V* __restrict__ psi, uint64_t chunks,
+                                                             uint32_t lane_limit, uint64_t x, int h,
+                                                             const PauliTermDev* __restrict__ terms, int nt,
                                                              double* __restrict__ partial) {
     __shared__ double sh[kExpThreads];
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t w0 = (uint64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * (kExpThreads / 64) +
+                                                                 (threadIdx.x >> 6));
+    const uint64_t W = (uint64_t)gridDim.x * (kExpThreads / 64);
+    constexpr int U = 4;
     double acc = 0.0;
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < half; k += stride) {
-        uint64_t i = insert_zero(k, h);
-        uint64_t j = i ^ x;
-        V a = psi[i], b = psi[j];
-        // z = conj(a) * b
-        double zr = (double)a.x * (double)b.x + (double)a.y * (double)b.y;
-        double zi = (double)a.x * (double)b.y - (double)a.y * (double)b.x;
-        double wir = 0.0, wii = 0.0, wjr = 0.0, wji = 0.0;
-        for (int t = 0; t < nt; ++t) {
-            PauliTermDev tm = terms[t];
-            bool si = __popcll(i & tm.zy) & 1;
-            bool sj = __popcll(j & tm.zy) & 1;
-            wir += si ? -tm.cr : tm.cr;
-            wii += si ? -tm.ci : tm.ci;
-            wjr += sj ? -tm.cr : tm.cr;
-            wji += sj ? -tm.ci : tm.ci;
+    if (lane < lane_limit) {
+        for (uint64_t c = w0; c < chunks; c += U * W) {
+            V a[U], b[U];
+            uint64_t ii[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ii[u] = insert_zero(((c + u * W) << 6) | lane, h);
+                if (c + u * W < chunks) {
+                    a[u] = psi[ii[u]];
+                    b[u] = psi[ii[u] ^ x];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (c + u * W >= chunks) break;
+                const uint64_t i = ii[u], j = i ^ x;
+                // z = conj(a) * b
+                const double zr = (double)a[u].x * (double)b[u].x + (double)a[u].y * (double)b[u].y;
+                const double zi = (double)a[u].x * (double)b[u].y - (double)a[u].y * (double)b[u].x;
+                double wir = 0.0, wii = 0.0, wjr = 0.0, wji = 0.0;
+                for (int t = 0; t < nt; ++t) {
+                    const PauliTermDev tm = terms[t];
+                    const bool si = __popcll(i & tm.zy) & 1;
+                    const bool sj = __popcll(j & tm.zy) & 1;
+                    wir += si ? -tm.cr : tm.cr;
+                    wii += si ? -tm.ci : tm.ci;
+                    wjr += sj ? -tm.cr : tm.cr;
+                    wji += sj ? -tm.ci : tm.ci;
+                }
+                // Re(z * w_i) + Re(conj(z) * w_j)
+                acc += zr * wir - zi * wii + zr * wjr + zi * wji;
+            }
         }
-        // Re(z * w_i) + Re(conj(z) * w_j)
-        acc += zr * wir - zi * wii + zr * wjr + zi * wji;
     }
     double s = block_sum_e<kExpThreads>(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -156,13 +223,6 @@ int compile_pauli_sum(int n, const uint8_t* paulis, const double* coeffs, int nu
     return LQ_OK;
 }
 
-static int exp_blocks(uint64_t work) {
-    uint64_t want = (work + kExpThreads * 4 - 1) / (kExpThreads * 4);
-    if (want < 1) want = 1;
-    if (want > kExpBlocksMax) want = kExpBlocksMax;
-    return (int)want;
-}
-
 template <typename V>
 static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
     const int ng = (int)ps.groups.size();
@@ -173,10 +233,39 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
     // device layout: [terms][partials...][result]
     size_t term_bytes = ps.terms.size() * sizeof(PauliTermDev);
     size_t term_pad = (term_bytes + 255) & ~(size_t)255;
-    int nb_diag = exp_blocks(st->dim);
-    int nb_flip = exp_blocks(st->dim >> 1);
+    const uint64_t chunks = st->dim >= 64 ? st->dim >> 6 : 1;
+    const uint32_t lane_limit = st->dim >= 64 ? 64u : (uint32_t)st->dim;
+    const uint32_t R = (uint32_t)std::min<uint64_t>(chunks, 64);
+    const uint64_t items = chunks / R;
+    const uint64_t rmask = (uint64_t)(R - 1) << 6;
+    int nb_diag = (int)std::min<uint64_t>(std::max<uint64_t>((items + 3) / 4, 1), kExpBlocksMax);
+    const uint64_t half = st->dim >> 1;
+    const uint64_t fchunks = half >= 64 ? half >> 6 : 1;
+    const uint32_t flanes = half >= 64 ? 64u : (uint32_t)half;
+    int nb_flip = (int)std::min<uint64_t>(std::max<uint64_t>((fchunks + 15) / 16, 1), kExpBlocksMax);
     int nb = std::max(nb_diag, nb_flip);
-    size_t part_count = (size_t)ng * nb;
+    // Device term table: the diagonal group's terms are reordered static
+    // first (masks that miss the r bits), then dynamic; a launch takes every
+    // static term and up to kDiagDyn dynamic ones, further launches the rest.
+    std::vector<PauliTermDev> dev = ps.terms;
+    int diag_first = -1, diag_static = 0, diag_count = 0;
+    size_t launches = 0;
+    for (const PauliGroup& grp : ps.groups) {
+        if (grp.x != 0) {
+            ++launches;
+            continue;
+        }
+        std::vector<PauliTermDev> st_, dy_;
+        for (int t = grp.first; t < grp.first + grp.count; ++t)
+            ((ps.terms[t].zy & rmask) == 0 ? st_ : dy_).push_back(ps.terms[t]);
+        std::copy(st_.begin(), st_.end(), dev.begin() + grp.first);
+        std::copy(dy_.begin(), dy_.end(), dev.begin() + grp.first + st_.size());
+        diag_first = grp.first;
+        diag_static = (int)st_.size();
+        diag_count = grp.count;
+        launches += std::max<size_t>(1, (dy_.size() + kDiagDyn - 1) / kDiagDyn);
+    }
+    size_t part_count = launches * nb;
     char* base = (char*)scratch(term_pad + (part_count + 8) * sizeof(double));
     if (!base) {
         set_error("scratch allocation failed");
@@ -187,20 +276,29 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
     double* d_out = d_part + part_count;
     double* h = host_staging(term_bytes / sizeof(double) + 16);
     if (!h) return LQ_ERR_OUT_OF_MEMORY;
-    std::memcpy(h, ps.terms.data(), term_bytes);
+    std::memcpy(h, dev.data(), term_bytes);
     LQ_HIP_CHECK(hipMemcpyAsync(d_terms, h, term_bytes, hipMemcpyHostToDevice, stream()));
     int written = 0;
     for (int g = 0; g < ng; ++g) {
         const PauliGroup& grp = ps.groups[g];
         if (grp.x == 0) {
-            hipLaunchKernelGGL(k_expect_diag<V>, dim3(nb_diag), dim3(kExpThreads), 0, stream(),
-                               (const V*)st->d_amps, st->dim, (const PauliTermDev*)(d_terms + grp.first), grp.count,
-                               d_part + written);
-            written += nb_diag;
+            // launch 0: the static terms and the first dynamic ones
+            const int nd_all = diag_count - diag_static;
+            int t0 = 0;
+            do {
+                const int ns = t0 == 0 ? diag_static : 0;
+                const int nd = std::min(kDiagDyn, nd_all - t0);
+                const int first = diag_first + (t0 == 0 ? 0 : diag_static + t0);
+                hipLaunchKernelGGL(k_expect_diag<V>, dim3(nb_diag), dim3(kExpThreads), 0, stream(),
+                                   (const V*)st->d_amps, items, R, lane_limit,
+                                   (const PauliTermDev*)(d_terms + first), ns, std::max(nd, 0), d_part + written);
+                written += nb_diag;
+                t0 += kDiagDyn;
+            } while (t0 < nd_all);
         } else {
             int hbit = 63 - __builtin_clzll(grp.x);
             hipLaunchKernelGGL(k_expect_flip<V>, dim3(nb_flip), dim3(kExpThreads), 0, stream(),
-                               (const V*)st->d_amps, st->dim >> 1, grp.x, hbit,
+                               (const V*)st->d_amps, fchunks, flanes, grp.x, hbit,
                                (const PauliTermDev*)(d_terms + grp.first), grp.count, d_part + written);
             written += nb_flip;
         }

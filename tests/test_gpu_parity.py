@@ -288,3 +288,22 @@ def test_large_state_round_trip(hip):
     lq._lib.call("lq_execute", st.handle, abi.gate_array(inv), len(inv), abi.EXEC_PER_GATE)
     p0 = st.inner_product(lq.State(n))
     assert abs(p0 - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("n", [3, 6, 9, 16])
+def test_diagonal_groups_over_32_terms(hip, n):
+    """Diagonal (I/Z-only) terms share one group; more than 32 of them run in
+    several passes, and the lane/chunk split of the sign must hold for states
+    shorter and longer than a wavefront's 64 amplitudes."""
+    say("diagonal group n=%d" % n)
+    rng = np.random.default_rng(4000 + n)
+    psi = bf.random_state(n, rng)
+    codes = np.where(rng.random((45, n)) < 0.5, 3, 0).astype(np.uint8)
+    codes[:5, :] = np.array([[1, 2] * n][0][:n], dtype=np.uint8)  # a few flip terms as well
+    coeffs = rng.normal(size=45)
+    st = lq.State(n)
+    st.set_vector(psi)
+    obs = lq.PauliObservable(n)
+    for r, c in zip(codes, coeffs):
+        obs.add_term(lq.PauliTerm(c, r))
+    assert abs(obs.expectation(st) - oracle.pauli_sum(psi, n, codes, coeffs)) <= 1e-10
