@@ -21,6 +21,7 @@
 #include <mutex>
 
 #include "lq_fused.h"
+#include "lq_tile_ops.h"
 
 namespace lq {
 
@@ -414,9 +415,6 @@ void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan
 // ---------------------------------------------------------------------------------
 constexpr int kTileThreads = 256;
 
-__device__ __forceinline__ uint32_t ins0(uint32_t k, int p) {
-    return ((k >> p) << (p + 1)) | (k & ((1u << p) - 1u));
-}
 
 template <typename V, typename T>
 __device__ __forceinline__ V cm(const double* m, int i, V a) {
@@ -613,202 +611,6 @@ __global__ void __launch_bounds__(tile_threads(K)) k_tile(V* __restrict__ psi, T
 // ---------------------------------------------------------------------------------
 // grouped kernel (tiles of >= 2^8 amplitudes)
 // ---------------------------------------------------------------------------------
-// LDS slot of tile element e: XOR-swizzles the low 4 bits with bits 4..7 so
-// that 16-amplitude subcubes over any 4 bits, read by consecutive lanes, fall
-// in different LDS banks.
-__device__ __forceinline__ uint32_t swz(uint32_t e) { return e ^ ((e >> 4) & 15u); }
-
-// One pair op on subcube position P.  `cin` lists the controls inside the
-// subcube: for a compile-time slot s the test is uniform (a scalar branch),
-// never a per-element select.
-template <int P, int KIND, typename T>
-__device__ __forceinline__ void g_pair(T* xr, T* xi, const TileOp& op, int cin) {
-    const T m0 = (T)op.m[0], m1 = (T)op.m[1], m2 = (T)op.m[2], m3 = (T)op.m[3];
-    const T m4 = (T)op.m[4], m5 = (T)op.m[5], m6 = (T)op.m[6], m7 = (T)op.m[7];
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        if (s & (1 << P)) continue;
-        if ((s & cin) != cin) continue;
-        const int t = s | (1 << P);
-        const T ar = xr[s], ai = xi[s], br = xr[t], bi = xi[t];
-        if (KIND == kTPairReal) {
-            xr[s] = m0 * ar + m2 * br;
-            xi[s] = m0 * ai + m2 * bi;
-            xr[t] = m4 * ar + m6 * br;
-            xi[t] = m4 * ai + m6 * bi;
-        } else if (KIND == kTPairRX) {
-            // [[c, i*m3], [i*m5, c']]: re/im of c*a + i*m3*b etc.
-            xr[s] = m0 * ar - m3 * bi;
-            xi[s] = m0 * ai + m3 * br;
-            xr[t] = m6 * br - m5 * ai;
-            xi[t] = m6 * bi + m5 * ar;
-        } else if (KIND == kTPairMat) {
-            xr[s] = (m0 * ar - m1 * ai) + (m2 * br - m3 * bi);
-            xi[s] = (m0 * ai + m1 * ar) + (m2 * bi + m3 * br);
-            xr[t] = (m4 * ar - m5 * ai) + (m6 * br - m7 * bi);
-            xi[t] = (m4 * ai + m5 * ar) + (m6 * bi + m7 * br);
-        } else if (KIND == kTPairSwap) {
-            xr[s] = br;
-            xi[s] = bi;
-            xr[t] = ar;
-            xi[t] = ai;
-        } else if (KIND == kTPairNegY) {
-            // a0' = i*a1, a1' = -i*a0 (single_qubit.rs:193-195)
-            xr[s] = -bi;
-            xi[s] = br;
-            xr[t] = ai;
-            xi[t] = -ar;
-        } else {
-            // s*(a0 + a1), s*(a0 - a1) (single_qubit.rs:248-249)
-            xr[s] = m0 * (ar + br);
-            xi[s] = m0 * (ai + bi);
-            xr[t] = m0 * (ar - br);
-            xi[t] = m0 * (ai - bi);
-        }
-    }
-}
-
-template <int KIND, typename T>
-__device__ __forceinline__ void g_pair_at(T* xr, T* xi, const TileOp& op, int cin) {
-    switch (op.sp0) {
-        case 0: g_pair<0, KIND, T>(xr, xi, op, cin); break;
-        case 1: g_pair<1, KIND, T>(xr, xi, op, cin); break;
-        case 2: g_pair<2, KIND, T>(xr, xi, op, cin); break;
-        default: g_pair<3, KIND, T>(xr, xi, op, cin); break;
-    }
-}
-
-// SWAP of subcube positions P0 (value 1) and P1 (value 0).
-template <int P0, int P1, typename T>
-__device__ __forceinline__ void g_swap2(T* xr, T* xi) {
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        if (!((s >> P0) & 1) || ((s >> P1) & 1)) continue;
-        const int t = s ^ (1 << P0) ^ (1 << P1);
-        T r = xr[s], i = xi[s];
-        xr[s] = xr[t];
-        xi[s] = xi[t];
-        xr[t] = r;
-        xi[t] = i;
-    }
-}
-
-// 4x4 block on subcube positions PA (bit a) and PB (bit b), sub-index
-// (bit a << 1) | bit b.
-template <int PA, int PB, typename T>
-__device__ __forceinline__ void g_quad(T* xr, T* xi, const double* m) {
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        if (((s >> PA) & 1) || ((s >> PB) & 1)) continue;
-        const int ix[4] = {s, s | (1 << PB), s | (1 << PA), s | (1 << PA) | (1 << PB)};
-        T ar[4], ai[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            ar[c] = xr[ix[c]];
-            ai[c] = xi[ix[c]];
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            T yr = (T)0, yi = (T)0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const T er = (T)m[8 * r + 2 * c], ei = (T)m[8 * r + 2 * c + 1];
-                yr += er * ar[c] - ei * ai[c];
-                yi += er * ai[c] + ei * ar[c];
-            }
-            xr[ix[r]] = yr;
-            xi[ix[r]] = yi;
-        }
-    }
-}
-
-template <bool RICH, typename T>
-__device__ __forceinline__ void g_apply(T* xr, T* xi, const TileOp& op, uint64_t gb) {
-    // controls outside the subcube are fixed for the whole subcube
-    if ((gb & op.cout) != op.cout) return;
-    const int kind = op.kind;
-    const int cin = op.cin;
-    if (kind == kTDiag) {
-        const int s0p = op.s0p, s1p = op.s1p;
-        const int tb0 = op.osel0 >= 0 ? (int)((gb >> op.osel0) & 1ull) : 0;
-        const int tb1 = op.osel1 >= 0 ? (int)((gb >> op.osel1) & 1ull) : 0;
-        const T r0 = (T)op.m[0], i0 = (T)op.m[1], r1 = (T)op.m[2], i1 = (T)op.m[3];
-        const T r2 = (T)op.m[4], i2 = (T)op.m[5], r3 = (T)op.m[6], i3 = (T)op.m[7];
-        if (s0p < 0 && s1p < 0) {
-            // selectors outside the subcube: one phase for the thread
-            const T pr = tb1 ? (tb0 ? r3 : r2) : (tb0 ? r1 : r0);
-            const T pi = tb1 ? (tb0 ? i3 : i2) : (tb0 ? i1 : i0);
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                if ((s & cin) != cin) continue;
-                const T yr = pr * xr[s] - pi * xi[s];
-                const T yi = pr * xi[s] + pi * xr[s];
-                xr[s] = yr;
-                xi[s] = yi;
-            }
-        } else if (s1p < 0) {
-            // first selector inside: two candidate phases per thread
-            const T p0r = tb1 ? r2 : r0, p0i = tb1 ? i2 : i0;
-            const T p1r = tb1 ? r3 : r1, p1i = tb1 ? i3 : i1;
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                if ((s & cin) != cin) continue;
-                const bool b = (s >> s0p) & 1;
-                const T pr = b ? p1r : p0r, pi = b ? p1i : p0i;
-                const T yr = pr * xr[s] - pi * xi[s];
-                const T yi = pr * xi[s] + pi * xr[s];
-                xr[s] = yr;
-                xi[s] = yi;
-            }
-        } else {
-            // general: second selector inside (first inside or out)
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                if ((s & cin) != cin) continue;
-                const int b0 = s0p >= 0 ? ((s >> s0p) & 1) : tb0;
-                const int b1 = (s >> s1p) & 1;
-                const T pr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
-                const T pi = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
-                const T yr = pr * xr[s] - pi * xi[s];
-                const T yi = pr * xi[s] + pi * xr[s];
-                xr[s] = yr;
-                xi[s] = yi;
-            }
-        }
-    } else if (kind == kTPairReal) {
-        g_pair_at<kTPairReal, T>(xr, xi, op, cin);
-    } else if (kind == kTPairRX) {
-        g_pair_at<kTPairRX, T>(xr, xi, op, cin);
-    } else if (kind == kTPairMat) {
-        g_pair_at<kTPairMat, T>(xr, xi, op, cin);
-    } else if (kind == kTPairSwap) {
-        g_pair_at<kTPairSwap, T>(xr, xi, op, cin);
-    } else if (kind == kTPairNegY) {
-        g_pair_at<kTPairNegY, T>(xr, xi, op, cin);
-    } else if (kind == kTPairH) {
-        g_pair_at<kTPairH, T>(xr, xi, op, cin);
-    } else if constexpr (RICH) {
-        const int code = op.sp0 * 4 + op.sp1;
-        if (kind == kTSwap2) {
-            switch (code) {
-#define LQ_SW(a, b) case a * 4 + b: g_swap2<a, b, T>(xr, xi); break;
-                LQ_SW(0, 1) LQ_SW(0, 2) LQ_SW(0, 3) LQ_SW(1, 0) LQ_SW(1, 2) LQ_SW(1, 3)
-                LQ_SW(2, 0) LQ_SW(2, 1) LQ_SW(2, 3) LQ_SW(3, 0) LQ_SW(3, 1) LQ_SW(3, 2)
-#undef LQ_SW
-                default: break;
-            }
-        } else {
-            switch (code) {
-#define LQ_QD(a, b) case a * 4 + b: g_quad<a, b, T>(xr, xi, op.m); break;
-                LQ_QD(0, 1) LQ_QD(0, 2) LQ_QD(0, 3) LQ_QD(1, 0) LQ_QD(1, 2) LQ_QD(1, 3)
-                LQ_QD(2, 0) LQ_QD(2, 1) LQ_QD(2, 3) LQ_QD(3, 0) LQ_QD(3, 1) LQ_QD(3, 2)
-#undef LQ_QD
-                default: break;
-            }
-        }
-    }
-}
-
 // At least two waves per SIMD: two 64 KiB tiles per CU are resident, so the
 // register budget is 256 VGPRs per lane.
 template <int K, bool RICH, typename V, typename T>
