@@ -304,6 +304,25 @@ int lq_set_batch_graphs(int enable);
 /* Graph captures and replays so far in this process. */
 int lq_batch_graph_stats(long long* captures, long long* replays);
 
+/* Run-time specialisation of fused tile passes (no reference counterpart).
+ * A pass kernel is generated for each pass structure (tile geometry, gate
+ * kinds, positions and control masks; not the angles), compiled with hipRTC
+ * and launched in place of the generic tile kernel.  LQ_JIT_REPEAT (the
+ * default) compiles a structure the second time a plan holds it, for passes
+ * of at least 1024 tiles; LQ_JIT_ALWAYS compiles every structure at first
+ * sight; LQ_JIT_OFF keeps the generic kernel.  The arithmetic is the same,
+ * so results do not depend on the mode. */
+#define LQ_JIT_OFF 0
+#define LQ_JIT_REPEAT 1
+#define LQ_JIT_ALWAYS 2
+int lq_set_jit(int mode);
+/* Kernels compiled, compiles or loads that failed (those passes stay on the
+ * generic kernel), specialised launches, and seconds spent compiling. */
+int lq_jit_stats(long long* compiled, long long* failed, long long* launches, double* compile_seconds);
+/* Compile the specialised kernels of a circuit's fused plan ahead of time
+ * (needs no device; loaded on first use).  kernels: kernels ready. */
+int lq_jit_compile_plan(int num_qubits, int dtype, const lq_gate* gates, int count, int* kernels);
+
 /* ---- Optimizers and VQE (src/optimization/optimizer.rs, vqe.rs) -------- */
 typedef double (*lq_objective_fn)(const double* x, int dim, void* user);
 typedef void (*lq_gradient_fn)(const double* x, int dim, double* grad, void* user);

@@ -19,5 +19,6 @@ from . import noise
 from .noise import (NoiseModel, NoiseParams, DepolarizingNoise, AmplitudeDampingNoise, PhaseDampingNoise,
                     ThermalRelaxationNoise, CompositeNoise)
 from .batch import batch_expectations
+from . import jit
 
 __version__ = "0.1.0"

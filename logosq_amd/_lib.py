@@ -75,6 +75,10 @@ SIGNATURES = {
                           c_double, P_double, P_double, P_double, P_int, P_int],
     "lq_set_batch_graphs": [c_int],
     "lq_batch_graph_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+    "lq_set_jit": [c_int],
+    "lq_jit_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                     ctypes.POINTER(ctypes.c_longlong), P_double],
+    "lq_jit_compile_plan": [c_int, c_int, P_gate, c_int, P_int],
     "lq_batch_expectations": [c_int, P_int, ctypes.POINTER(P_gate), P_int, ctypes.POINTER(P_u8),
                               ctypes.POINTER(P_double), P_int, c_int, c_int, P_double],
     "lq_gd_minimize": [c_double, c_int, c_double, OBJ_FN, GRAD_FN, c_void_p, P_double, c_int, P_double, c_void_p],

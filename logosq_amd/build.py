@@ -30,7 +30,27 @@ def sources():
 # Per-file flags.  lq_fused.hip: LLVM's common-code sinking would merge the
 # per-position copies of the register subcube code into one block with a
 # computed array index, which sends the subcube to scratch memory.
-EXTRA = {"lq_fused.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+EXTRA = {"lq_fused.hip": ["-mllvm", "-simplifycfg-sink-common=false"], "lq_jit.hip": ["-I", BUILD]}
+
+# Headers the run-time specialised kernels are compiled from (hipRTC), embedded
+# in the library as text.
+JIT_HEADERS = [("kTileTypesSrc", "lq_tile_types.h"), ("kTileOpsSrc", "lq_tile_ops.h")]
+JIT_INC = os.path.join(BUILD, "lq_jit_src.inc")
+
+
+def write_jit_sources():
+    """Write _build/lq_jit_src.inc: each header as a C++ raw string literal
+    (rewritten only when the text changes, so incremental builds hold)."""
+    out = []
+    for var, name in JIT_HEADERS:
+        text = open(os.path.join(CSRC, name)).read()
+        if ')LQSRC"' in text:
+            raise RuntimeError("%s holds the raw-string delimiter" % name)
+        out.append('static const char %s[] = R"LQSRC(%s)LQSRC";\n' % (var, text))
+    text = "// generated by build.py from csrc headers; do not edit\n" + "".join(out)
+    if not os.path.exists(JIT_INC) or open(JIT_INC).read() != text:
+        with open(JIT_INC, "w") as f:
+            f.write(text)
 
 
 def _compile(src, incremental=False):
@@ -51,11 +71,12 @@ def build(verbose=True, incremental=False):
     """Compile every source (incremental=True: only sources newer than their
     object file or than any header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
+    write_jit_sources()
     srcs = sources()
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))

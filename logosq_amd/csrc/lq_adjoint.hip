@@ -758,6 +758,7 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
             set_error("adjoint gradient: renormalising step in the forward plan");
             return LQ_ERR_UNSUPPORTED;
         }
+    jit_prepare(plf, dtype, (uint64_t)count);
     const uint64_t dim = 1ull << n;
     const size_t amp = sizeof(V);
     const uint64_t nwp = (dim >> Kb) * kAWaves;
@@ -850,7 +851,7 @@ static int adjoint_batch_t(int n, int dtype, const lq_gate* tmpl, int num_gates,
         hipLaunchKernelGGL(k_init_states<V>, dim3(grid_of(total)), dim3(kAThreads), 0, stream(), psi, dim, total, 1);
         LQ_LAUNCH_CHECK();
         for (const Step& st : plf.steps) {
-            rc = launch_tile_pass(psi, dtype, st.pass, d_of, d_gr, cn, dim, nof);
+            rc = launch_tile_pass(psi, dtype, st.pass, d_of, d_gr, cn, dim, nof, st.jit);
             if (rc) return rc;
         }
         // lambda = H psi, E = Re <psi|lambda>

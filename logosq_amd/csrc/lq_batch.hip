@@ -303,7 +303,7 @@ static int job_launch(const BatchJob& j, void* psi) {
             }
             continue;
         }
-        rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)j.ninst, j.dim, j.nops);
+        rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)j.ninst, j.dim, j.nops, s.jit);
         if (rc) return rc;
     }
     if (j.blocks > 0) {
@@ -357,6 +357,7 @@ static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
         k.push_back(((uint64_t)(uint32_t)p.group_first << 32) | (uint32_t)p.group_count);
         k.push_back(((uint64_t)(uint32_t)p.rich << 32));
         k.push_back(p.ntiles);
+        k.push_back((uint64_t)(uintptr_t)s.jit);
     }
     return k;
 }
@@ -469,6 +470,7 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
     if (rc) return rc;
     Plan plan;
     plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+    jit_prepare(plan, dtype, (uint64_t)ninst);
     std::vector<int> fp = first_prims(base, num_gates);
     // which gates use each parameter slot
     std::vector<std::vector<int>> users(num_params);
@@ -598,6 +600,7 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
         if (rc) return rc;
         Plan plan;
         plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+        jit_prepare(plan, dtype, (uint64_t)members.size());
         std::vector<int> fp = first_prims(base, ng);
         const size_t nops = plan.ops.size();
         const int total = (int)members.size();

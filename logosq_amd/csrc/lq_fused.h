@@ -22,9 +22,12 @@ struct Prim {
     int32_t mclass;     // kTPairReal for RY/CRY, kTPairRX for RX/CRX, else kTPairMat
 };
 
+struct JitKernel;  // a pass kernel compiled at run time for one pass structure
+
 struct Step {
     bool normalize;     // true: a State::normalize step, else a tile pass
     TilePass pass;
+    const JitKernel* jit = nullptr;  // specialised kernel for this pass, if any
 };
 
 struct Plan {
@@ -48,6 +51,18 @@ int default_low_bits(int n, int dtype);
 // Launch one tile pass over `ninst` states laid out `inst_stride` amplitudes
 // apart, with per-instance op tables `ops_stride` ops apart.
 int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, const TileGroup* d_groups,
-                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride);
+                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride, const JitKernel* jit = nullptr);
+
+// Attach specialised kernels to the plan's passes (Step::jit) as the
+// run-time specialisation policy allows, compiling new pass structures;
+// ninst: instances each pass will run over.  Does nothing inside a stream
+// capture.
+void jit_prepare(Plan& plan, int dtype, uint64_t ninst);
+// Compile (unless the mode is off) and load the specialised kernels a fused
+// execution of these gates will use, so that a timed run that follows does
+// not compile.
+int jit_warm(int n, int dtype, const lq_gate* gates, int count);
+int launch_jit(const JitKernel* k, void* psi, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
+               uint64_t inst_stride, uint64_t ops_stride, unsigned grid);
 
 }  // namespace lq

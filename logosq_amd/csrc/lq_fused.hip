@@ -745,12 +745,13 @@ static int dispatch_k(int K, void* psi, const TilePass& p, const TileOp* ops, co
 }
 
 int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_ops, const TileGroup* d_groups,
-                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride) {
+                     uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride, const JitKernel* jit) {
     const int K = p.L + p.hb;
     const size_t amp = dtype == LQ_C128 ? 16 : 8;
     const size_t lds = (amp << K) + (sizeof(uint64_t) << p.hb);
     uint64_t total = ninst * p.ntiles;
     unsigned grid = (unsigned)std::min<uint64_t>(total, 4096);
+    if (jit) return launch_jit(jit, psi, p, d_ops, ninst, inst_stride, ops_stride, grid);
     int rc;
     if (dtype == LQ_C128)
         rc = dispatch_k<double2, double>(K, psi, p, d_ops, d_groups, ninst, inst_stride, ops_stride, grid, lds);
@@ -795,7 +796,7 @@ int upload_plan(const std::vector<TileOp>& ops, const std::vector<TileGroup>& gr
 int run_plan(lq_state* st, const Plan& plan, const TileOp* d_ops, const TileGroup* d_groups) {
     for (const Step& s : plan.steps) {
         int rc = s.normalize ? normalize_device(st, nullptr)
-                             : launch_tile_pass(st->d_amps, st->dtype, s.pass, d_ops, d_groups, 1, 0, 0);
+                             : launch_tile_pass(st->d_amps, st->dtype, s.pass, d_ops, d_groups, 1, 0, 0, s.jit);
         if (rc) return rc;
     }
     return LQ_OK;
@@ -808,6 +809,7 @@ int execute_fused(lq_state* st, const lq_gate* gates, int count) {
     Plan plan;
     plan_passes(st->n, prims, default_tile_bits(st->n, st->dtype), default_low_bits(st->n, st->dtype), plan);
     if (plan.ops.empty() && plan.steps.empty()) return LQ_OK;
+    jit_prepare(plan, st->dtype, 1);
     TileOp* d_ops = nullptr;
     TileGroup* d_groups = nullptr;
     rc = upload_plan(plan.ops, plan.groups, &d_ops, &d_groups);

@@ -1,0 +1,314 @@
+// Tile-pass kernels specialised at run time.
+//
+// The generic grouped kernel (k_tileg) reads every op's kind, subcube
+// position and control masks at run time, so each op costs the selects and
+// path merges that cover all of its cases.  Here the host writes one kernel
+// per pass *structure* (tile geometry, group positions, and each op's kind,
+// positions and control masks; not the matrices), compiles it with hipRTC
+// and launches it in place of k_tileg: every op becomes straight-line
+// arithmetic on fixed registers.  Matrices still come from the op table, so
+// one compiled kernel serves every parameter value and every instance of a
+// batch.
+//
+// Policy (lq_set_jit): off; "repeat" (default) compiles a structure the
+// second time a plan holds it, and only for passes of at least
+// kJitMinTiles tiles (smaller passes are launch-bound); "always" compiles
+// everything at first sight.  Compiles of one plan run in parallel threads.
+// A failed compile leaves that structure on the generic kernel for good.
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "lq_fused.h"
+#include "lq_host.h"
+
+// kTileTypesSrc / kTileOpsSrc: the text of lq_tile_types.h / lq_tile_ops.h,
+// generated by the build next to the object files.
+#include "lq_jit_src.inc"
+
+namespace lq {
+
+struct JitKernel {
+    int seen = 0;
+    bool failed = false;
+    std::vector<char> code;  // code object from hipRTC
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+};
+
+static int g_jit_mode = LQ_JIT_REPEAT;
+static constexpr uint64_t kJitMinTiles = 1024;
+static std::unordered_map<std::string, std::unique_ptr<JitKernel>> g_jit;
+static long long g_jit_compiled = 0, g_jit_failed = 0, g_jit_launches = 0;
+static double g_jit_seconds = 0.0;
+
+static constexpr int jit_threads(int K) { return K <= 12 ? 256 : (1 << (K - 4)); }
+
+// Kernel text for one pass: constants K, HB, the groups' subcube positions
+// and every op's kind, positions and masks.  Op i of the pass reads its
+// matrix from ops[i].
+static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype) {
+    const int K = p.L + p.hb;
+    std::ostringstream s;
+    s << "#include \"lq_tile_ops.h\"\nusing namespace lq;\n";
+    if (dtype == LQ_C128)
+        s << "typedef double T;\ntypedef double V __attribute__((ext_vector_type(2)));\n";
+    else
+        s << "typedef float T;\ntypedef float V __attribute__((ext_vector_type(2)));\n";
+    s << "constexpr int K = " << K << ", HB = " << p.hb << ", NT = " << jit_threads(K) << ";\n";
+    s << "extern \"C\" __global__ void __launch_bounds__(NT, 2) lq_pass(V* __restrict__ psi, TilePass p, "
+         "const TileOp* __restrict__ ops, uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride) {\n";
+    s << "  __shared__ __attribute__((aligned(16))) V tile[1u << K];\n";
+    s << "  __shared__ uint64_t hoff[1u << HB];\n";
+    s << "  j_pass_begin<K, HB, NT>(hoff, p);\n";
+    s << "  const uint64_t total = ninst * p.ntiles;\n";
+    s << "  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {\n";
+    s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));\n";
+    s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
+    s << "    V* st = psi + (uint64_t)inst * inst_stride;\n";
+    s << "    const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;\n";
+    s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
+    for (int gi = 0; gi < p.group_count; ++gi) {
+        const TileGroup& G = groups[p.group_first + gi];
+        s << "    j_group<K, HB, NT, " << G.spos[0] << ", " << G.spos[1] << ", " << G.spos[2] << ", " << G.spos[3]
+          << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
+        for (int o = G.op_first; o < G.op_first + G.op_count; ++o) {
+            const TileOp& t = ops[p.op_first + o];
+            s << "      ";
+            if (t.cout) s << "if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) ";
+            switch (t.kind) {
+                case kTDiag: {
+                    auto tb = [&](int osel) {
+                        std::string r = "0";
+                        if (osel >= 0) r = "(int)((gb >> " + std::to_string(osel) + ") & 1ull)";
+                        return r;
+                    };
+                    s << "j_diag<" << t.s0p << ", " << t.s1p << ", " << t.cin << ", T>(xr, xi, op[" << o << "], "
+                      << tb(t.osel0) << ", " << tb(t.osel1) << ");\n";
+                    break;
+                }
+                case kTSwap2:
+                    s << "g_swap2<" << t.sp0 << ", " << t.sp1 << ", T>(xr, xi);\n";
+                    break;
+                case kTQuad:
+                    s << "g_quad<" << t.sp0 << ", " << t.sp1 << ", T>(xr, xi, op[" << o << "].m);\n";
+                    break;
+                default:
+                    s << "j_pair<" << t.sp0 << ", " << t.kind << ", " << t.cin << ", T>(xr, xi, op[" << o << "]);\n";
+                    break;
+            }
+        }
+        s << "    });\n";
+    }
+    s << "    j_tile_store<K, HB, NT>(tile, st, base, hoff);\n";
+    s << "  }\n}\n";
+    return s.str();
+}
+
+static const char* kJitOpts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=fast",
+                                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                 "-mllvm", "-simplifycfg-sink-common=false"};
+
+// Compile one kernel text; false (with the log on stderr when LQ_JIT_LOG is
+// set) on failure.
+static bool jit_compile(const std::string& src, std::vector<char>& code) {
+    if (const char* dir = std::getenv("LQ_JIT_DUMP")) {
+        // the kernel text, for reading (LQ_JIT_DUMP=<directory>)
+        const std::string path = std::string(dir) + "/lq_pass_" + std::to_string(std::hash<std::string>{}(src)) + ".hip";
+        if (FILE* f = std::fopen(path.c_str(), "w")) {
+            std::fputs(src.c_str(), f);
+            std::fclose(f);
+        }
+    }
+    const char* headers[] = {kTileTypesSrc, kTileOpsSrc};
+    const char* names[] = {"lq_tile_types.h", "lq_tile_ops.h"};
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "lq_pass.hip", 2, headers, names) != HIPRTC_SUCCESS) return false;
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(kJitOpts) / sizeof(kJitOpts[0])), kJitOpts);
+    bool ok = r == HIPRTC_SUCCESS;
+    if (!ok && std::getenv("LQ_JIT_LOG")) {
+        size_t n = 0;
+        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+            std::string log(n, '\0');
+            if (hiprtcGetProgramLog(prog, &log[0]) == HIPRTC_SUCCESS) std::fprintf(stderr, "%s\n", log.c_str());
+        }
+    }
+    size_t n = 0;
+    if (ok) ok = hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS && n > 0;
+    if (ok) {
+        code.resize(n);
+        ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    return ok;
+}
+
+// Compile the given entries in parallel (at most 16 threads).
+static void compile_all(const std::vector<std::pair<const std::string*, JitKernel*>>& todo) {
+    if (todo.empty()) return;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<char> ok(todo.size(), 0);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nth = std::min<size_t>({todo.size(), (size_t)hw, (size_t)16});
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nth; ++t)
+        th.emplace_back([&, t] {
+            for (size_t i = t; i < todo.size(); i += nth) ok[i] = jit_compile(*todo[i].first, todo[i].second->code);
+        });
+    for (auto& x : th) x.join();
+    for (size_t i = 0; i < todo.size(); ++i) {
+        if (ok[i]) {
+            ++g_jit_compiled;
+        } else {
+            todo[i].second->failed = true;
+            todo[i].second->code.clear();
+            ++g_jit_failed;
+        }
+    }
+    g_jit_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Load a compiled code object on the current device.
+static bool jit_load(JitKernel* k) {
+    if (k->fn) return true;
+    if (k->failed || k->code.empty()) return false;
+    if (hipModuleLoadData(&k->mod, k->code.data()) != hipSuccess ||
+        hipModuleGetFunction(&k->fn, k->mod, "lq_pass") != hipSuccess) {
+        (void)hipGetLastError();
+        k->failed = true;
+        k->fn = nullptr;
+        ++g_jit_failed;
+        return false;
+    }
+    return true;
+}
+
+static bool jit_eligible(const TilePass& p, uint64_t ninst, bool force) {
+    const int K = p.L + p.hb;
+    if (K < 8 || K > 13 || p.group_count <= 0) return false;
+    return force || p.ntiles * ninst >= kJitMinTiles;
+}
+
+// Attach specialised kernels to a plan's passes (Step::jit), compiling the
+// structures the policy selects.  load=false: compile only (no device).
+// compile_now: compile every eligible structure at once, whatever the mode
+// (explicit requests); any_size: ignore the minimum pass size.
+static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bool compile_now, bool any_size) {
+    for (Step& s : plan.steps) s.jit = nullptr;
+    if (g_jit_mode == LQ_JIT_OFF && !compile_now) return 0;
+    const bool always = compile_now || g_jit_mode == LQ_JIT_ALWAYS;
+    std::vector<std::pair<Step*, JitKernel*>> use;
+    std::vector<std::pair<const std::string*, JitKernel*>> todo;
+    for (Step& s : plan.steps) {
+        if (s.normalize || !jit_eligible(s.pass, ninst, any_size || g_jit_mode == LQ_JIT_ALWAYS)) continue;
+        std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype);
+        auto it = g_jit.find(src);
+        if (it == g_jit.end()) it = g_jit.emplace(std::move(src), std::make_unique<JitKernel>()).first;
+        JitKernel* k = it->second.get();
+        ++k->seen;
+        if (k->failed) continue;
+        if (!k->fn && k->code.empty() && (always || k->seen >= 2) &&
+            std::find_if(todo.begin(), todo.end(), [&](const auto& e) { return e.second == k; }) == todo.end())
+            todo.emplace_back(&it->first, k);
+        use.emplace_back(&s, k);
+    }
+    compile_all(todo);
+    int ready = 0;
+    for (auto& [s, k] : use) {
+        if (k->code.empty() && !k->fn) continue;
+        if (load && !jit_load(k)) continue;
+        if (load) s->jit = k;
+        ++ready;
+    }
+    return ready;
+}
+
+void jit_prepare(Plan& plan, int dtype, uint64_t ninst) {
+    // never inside a stream capture: compiling and loading are not stream work
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream(), &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        for (Step& s : plan.steps) s.jit = nullptr;
+        return;
+    }
+    (void)jit_prepare_impl(plan, dtype, ninst, true, false, false);
+}
+
+int jit_warm(int n, int dtype, const lq_gate* gates, int count) {
+    if (g_jit_mode == LQ_JIT_OFF) return LQ_OK;
+    std::vector<Prim> prims;
+    int rc = gates_to_prims(n, gates, count, prims);
+    if (rc) return rc;
+    Plan plan;
+    plan_passes(n, prims, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+    (void)jit_prepare_impl(plan, dtype, 1, true, true, false);
+    return LQ_OK;
+}
+
+int launch_jit(const JitKernel* k, void* psi, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
+               uint64_t inst_stride, uint64_t ops_stride, unsigned grid) {
+    const int K = p.L + p.hb;
+    TilePass pp = p;
+    void* args[] = {&psi, &pp, &d_ops, &ninst, &inst_stride, &ops_stride};
+    hipError_t e = hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)jit_threads(K), 1, 1, 0, stream(), args,
+                                         nullptr);
+    if (e != hipSuccess) {
+        set_error("specialised tile pass launch failed: %s", hipGetErrorString(e));
+        return LQ_ERR_DEVICE;
+    }
+    ++g_jit_launches;
+    return LQ_OK;
+}
+
+}  // namespace lq
+
+using namespace lq;
+
+extern "C" int lq_set_jit(int mode) {
+    clear_error();
+    if (mode != LQ_JIT_OFF && mode != LQ_JIT_REPEAT && mode != LQ_JIT_ALWAYS) {
+        set_error("jit mode must be LQ_JIT_OFF, LQ_JIT_REPEAT or LQ_JIT_ALWAYS");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    g_jit_mode = mode;
+    return LQ_OK;
+}
+
+extern "C" int lq_jit_stats(long long* compiled, long long* failed, long long* launches, double* compile_seconds) {
+    clear_error();
+    if (compiled) *compiled = g_jit_compiled;
+    if (failed) *failed = g_jit_failed;
+    if (launches) *launches = g_jit_launches;
+    if (compile_seconds) *compile_seconds = g_jit_seconds;
+    return LQ_OK;
+}
+
+extern "C" int lq_jit_compile_plan(int num_qubits, int dtype, const lq_gate* gates, int count, int* kernels) {
+    clear_error();
+    if (num_qubits < 1 || num_qubits > kMaxQubits || count < 0 || (count > 0 && !gates) ||
+        (dtype != LQ_C128 && dtype != LQ_C64)) {
+        set_error("invalid arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<Prim> prims;
+    int rc = gates_to_prims(num_qubits, gates, count, prims);
+    if (rc) return rc;
+    Plan plan;
+    plan_passes(num_qubits, prims, default_tile_bits(num_qubits, dtype), default_low_bits(num_qubits, dtype), plan);
+    const long long before = g_jit_failed;
+    const int ready = jit_prepare_impl(plan, dtype, 1, false, true, true);
+    if (kernels) *kernels = ready;
+    if (g_jit_failed != before) {
+        set_error("hipRTC failed on %lld pass kernel(s) (LQ_JIT_LOG=1 prints the log)", g_jit_failed - before);
+        return LQ_ERR_DEVICE;
+    }
+    return LQ_OK;
+}

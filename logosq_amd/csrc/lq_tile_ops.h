@@ -281,4 +281,100 @@ __device__ __forceinline__ void j_diag(T* xr, T* xi, const TileOp& op, int tb0, 
     }
 }
 
+// ---- pass skeleton of the specialised kernels --------------------------------
+// Same tile walk as the generic kernel, with the tile geometry (K tile bits,
+// HB of them high) fixed at compile time.
+__device__ __forceinline__ uint64_t j_ins0(uint64_t k, int p) {
+    return ((k >> p) << (p + 1)) | (k & ((1ull << p) - 1ull));
+}
+
+// Global offsets of the 2^HB high-bit patterns of a tile, into LDS.
+template <int K, int HB, int NT>
+__device__ __forceinline__ void j_pass_begin(uint64_t* hoff, const TilePass& p) {
+    for (uint32_t h = threadIdx.x; h < (1u << HB); h += NT) {
+        uint64_t o = 0;
+        for (int j = 0; j < HB; ++j)
+            if ((h >> j) & 1u) o |= 1ull << p.hpos[j];
+        hoff[h] = o;
+    }
+    __syncthreads();
+}
+
+// Global index of tile t's element 0.
+template <int K, int HB>
+__device__ __forceinline__ uint64_t j_tile_base(uint64_t t, const TilePass& p) {
+    uint64_t base = t << (K - HB);
+#pragma unroll
+    for (int j = 0; j < HB; ++j) base = j_ins0(base, p.hpos[j]);
+    return base;
+}
+
+template <int K, int HB>
+__device__ __forceinline__ uint64_t j_gidx(uint64_t base, const uint64_t* hoff, uint32_t e) {
+    constexpr int L = K - HB;
+    return base | (e & ((1u << L) - 1u)) | hoff[e >> L];
+}
+
+template <int K, int HB, int NT, typename V>
+__device__ __forceinline__ void j_tile_load(V* tile, const V* __restrict__ s, uint64_t base, const uint64_t* hoff) {
+    constexpr int EPT = (1 << K) / NT;
+    const uint32_t tid = threadIdx.x;
+    V r[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) r[j] = s[j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT)];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) tile[swz(tid + (uint32_t)j * NT)] = r[j];
+    __syncthreads();
+}
+
+template <int K, int HB, int NT, typename V>
+__device__ __forceinline__ void j_tile_store(const V* tile, V* __restrict__ s, uint64_t base, const uint64_t* hoff) {
+    constexpr int EPT = (1 << K) / NT;
+    const uint32_t tid = threadIdx.x;
+    V r[EPT];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) r[j] = tile[swz(tid + (uint32_t)j * NT)];
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) s[j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT)] = r[j];
+    __syncthreads();
+}
+
+// One group: every thread gathers its 16-amplitude subcube over tile bits
+// S0..S3 into registers, runs body(xr, xi, gb) (gb: the subcube's global
+// base index) and scatters it back.
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, typename T, typename V, typename F>
+__device__ __forceinline__ void j_group(V* tile, uint64_t base, const uint64_t* hoff, F&& body) {
+    constexpr uint32_t NSUB = (1u << K) >> 4;
+    constexpr bool SFULL = NSUB >= (uint32_t)NT;
+    constexpr int SPT = SFULL ? (int)(NSUB / NT) : 1;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+        const uint32_t q = tid + (uint32_t)k * NT;
+        if (!SFULL && q >= NSUB) break;
+        const uint32_t eb = ins0(ins0(ins0(ins0(q, S0), S1), S2), S3);
+        const uint64_t gb = j_gidx<K, HB>(base, hoff, eb);
+        T xr[16], xi[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint32_t e = eb | ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                               ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+            const V v = tile[swz(e)];
+            xr[c] = v.x;
+            xi[c] = v.y;
+        }
+        body(xr, xi, gb);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint32_t e = eb | ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                               ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+            V v;
+            v.x = xr[c];
+            v.y = xi[c];
+            tile[swz(e)] = v;
+        }
+    }
+    __syncthreads();
+}
+
 }  // namespace lq

@@ -118,6 +118,9 @@ extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t repor
             if (dtype == LQ_C128) rc = set_fused_config_internal(tb, t0s, lows[li]);
             else rc = set_fused_config_internal(t0c, tb, lows[li]);
             if (rc) break;
+            // the kernels this geometry runs are compiled before timing
+            rc = jit_warm(n, dtype, circ.data(), (int)circ.size());
+            if (rc) break;
             float ms = 0.f;
             for (int rep_i = 0; rep_i < 2 && !rc; ++rep_i) {
                 (void)hipEventRecord(e0, stream());
