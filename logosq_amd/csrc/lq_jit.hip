@@ -79,10 +79,54 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
     for (int gi = 0; gi < p.group_count; ++gi) {
         const TileGroup& G = groups[p.group_first + gi];
+        // X gates with their controls outside the subcube and nothing but
+        // other such gates on their bit before (after) them in the group are
+        // done by flipping the gather (scatter) slots of the threads whose
+        // controls are set.
+        const int o0 = G.op_first, o1 = G.op_first + G.op_count;
+        std::vector<int> fold(G.op_count, 0);  // 1: gather, 2: scatter
+        auto touch = [&](const TileOp& t) -> int {
+            int m = t.cin;
+            if (t.kind == kTDiag) {
+                if (t.s0p >= 0) m |= 1 << t.s0p;
+                if (t.s1p >= 0) m |= 1 << t.s1p;
+            } else {
+                m |= 1 << t.sp0;
+                if (t.kind == kTSwap2 || t.kind == kTQuad) m |= 1 << t.sp1;
+            }
+            return m;
+        };
+        auto foldable = [&](const TileOp& t) { return t.kind == kTPairSwap && t.cin == 0 && t.cout != 0; };
+        int seen = 0;
+        for (int o = o0; o < o1; ++o) {
+            const TileOp& t = ops[p.op_first + o];
+            if (foldable(t) && !(seen & (1 << t.sp0))) fold[o - o0] = 1;
+            else seen |= touch(t);
+        }
+        seen = 0;
+        for (int o = o1 - 1; o >= o0; --o) {
+            const TileOp& t = ops[p.op_first + o];
+            if (fold[o - o0]) continue;
+            if (foldable(t) && !(seen & (1 << t.sp0))) fold[o - o0] = 2;
+            else seen |= touch(t);
+        }
+        auto flips = [&](int which) {
+            std::ostringstream f;
+            f << "[&](uint64_t gb) -> uint32_t { uint32_t f = 0u;";
+            for (int o = o0; o < o1; ++o) {
+                if (fold[o - o0] != which) continue;
+                const TileOp& t = ops[p.op_first + o];
+                f << " if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) f ^= "
+                  << (1u << G.spos[t.sp0]) << "u;";
+            }
+            f << " return f; }";
+            return f.str();
+        };
         s << "    j_group<K, HB, NT, " << G.spos[0] << ", " << G.spos[1] << ", " << G.spos[2] << ", " << G.spos[3]
           << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
         for (int o = G.op_first; o < G.op_first + G.op_count; ++o) {
             const TileOp& t = ops[p.op_first + o];
+            if (fold[o - o0]) continue;
             s << "      ";
             if (t.cout) s << "if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) ";
             switch (t.kind) {
@@ -107,7 +151,7 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
                     break;
             }
         }
-        s << "    });\n";
+        s << "    }, " << flips(1) << ", " << flips(2) << ");\n";
     }
     s << "    j_tile_store<K, HB, NT>(tile, st, base, hoff);\n";
     s << "  }\n}\n";

@@ -341,37 +341,51 @@ __device__ __forceinline__ void j_tile_store(const V* tile, V* __restrict__ s, u
 
 // One group: every thread gathers its 16-amplitude subcube over tile bits
 // S0..S3 into registers, runs body(xr, xi, gb) (gb: the subcube's global
-// base index) and scatters it back.
-template <int K, int HB, int NT, int S0, int S1, int S2, int S3, typename T, typename V, typename F>
-__device__ __forceinline__ void j_group(V* tile, uint64_t base, const uint64_t* hoff, F&& body) {
+// base index) and scatters it back.  gflip(gb) / sflip(gb): tile bits by
+// which the thread's gather / scatter slots are flipped (X gates whose
+// control lies outside the subcube, applied by addressing when they are the
+// first / last op of the group on their bit).
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, typename T, typename V, typename F, typename GF,
+          typename SF>
+__device__ __forceinline__ void j_group(V* tile, uint64_t base, const uint64_t* hoff, F&& body, GF&& gflip,
+                                        SF&& sflip) {
     constexpr uint32_t NSUB = (1u << K) >> 4;
     constexpr bool SFULL = NSUB >= (uint32_t)NT;
     constexpr int SPT = SFULL ? (int)(NSUB / NT) : 1;
-    const uint32_t tid = threadIdx.x;
+    uint32_t tid = threadIdx.x;
+    // Opaque to the optimiser, so the slot addresses below are formed in
+    // every tile iteration: hoisted out of the tile loop, every group's 16
+    // addresses would stay live across the whole pass and spill.
+    asm volatile("" : "+v"(tid));
 #pragma unroll
     for (int k = 0; k < SPT; ++k) {
         const uint32_t q = tid + (uint32_t)k * NT;
         if (!SFULL && q >= NSUB) break;
         const uint32_t eb = ins0(ins0(ins0(ins0(q, S0), S1), S2), S3);
         const uint64_t gb = j_gidx<K, HB>(base, hoff, eb);
+        // swz is linear over XOR and the slot bits are disjoint from eb's:
+        // slot c sits at swz(eb) ^ swz(offset of c), one XOR per slot
+        const uint32_t sb = swz(eb);
+        const uint32_t sg = sb ^ swz(gflip(gb));
         T xr[16], xi[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            const uint32_t e = eb | ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
-                               ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
-            const V v = tile[swz(e)];
+            const uint32_t oc = ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                                ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+            const V v = tile[sg ^ swz(oc)];
             xr[c] = v.x;
             xi[c] = v.y;
         }
         body(xr, xi, gb);
+        const uint32_t ss = sb ^ swz(sflip(gb));
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-            const uint32_t e = eb | ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
-                               ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+            const uint32_t oc = ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                                ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
             V v;
             v.x = xr[c];
             v.y = xi[c];
-            tile[swz(e)] = v;
+            tile[ss ^ swz(oc)] = v;
         }
     }
     __syncthreads();
