@@ -56,19 +56,106 @@ static constexpr int jit_threads(int K) { return K <= 12 ? 256 : (1 << (K - 4));
 // Kernel text for one pass: constants K, HB, the groups' subcube positions
 // and every op's kind, positions and masks.  Op i of the pass reads its
 // matrix from ops[i].
+//
+// Within a group, two rewrites (both exact up to rounding):
+// * X gates with their controls outside the subcube and nothing but other
+//   such gates on their bit before (after) them in the group are done by
+//   flipping the gather (scatter) slots of the threads whose controls are
+//   set, instead of a per-lane register permutation.
+// * Diagonal ops that depend on subcube bits only are moved past the
+//   non-diagonal ops that do not act on those bits, and each run of two or
+//   more becomes one 16-entry phase table per tile: sixteen threads form the
+//   products (in f64) into LDS while the tile loads, and the group applies
+//   the whole run with one complex multiply per amplitude.
+static constexpr int kMaxRuns = 32;
+
+struct GroupPlan {
+    std::vector<int> fold;            // per op of the group: 0, 1 gather, 2 scatter
+    std::vector<std::pair<int, int>> items;  // (0, op index) or (1, run id), in order
+};
+
 static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype) {
     const int K = p.L + p.hb;
+    const int NT = jit_threads(K);
+    auto touch = [&](const TileOp& t) -> int {
+        int m = t.cin;
+        if (t.kind == kTDiag) {
+            if (t.s0p >= 0) m |= 1 << t.s0p;
+            if (t.s1p >= 0) m |= 1 << t.s1p;
+        } else {
+            m |= 1 << t.sp0;
+            if (t.kind == kTSwap2 || t.kind == kTQuad) m |= 1 << t.sp1;
+        }
+        return m;
+    };
+    auto foldable = [&](const TileOp& t) { return t.kind == kTPairSwap && t.cin == 0 && t.cout != 0; };
+    // diagonal and depending on subcube bits only
+    auto local_diag = [&](const TileOp& t) {
+        return t.kind == kTDiag && t.cout == 0 && t.osel0 < 0 && t.osel1 < 0;
+    };
+    std::vector<std::vector<int>> runs;  // op indices (pass-relative) of each merged run
+    std::vector<GroupPlan> gp(p.group_count);
+    for (int gi = 0; gi < p.group_count; ++gi) {
+        const TileGroup& G = groups[p.group_first + gi];
+        const int o0 = G.op_first, o1 = G.op_first + G.op_count;
+        GroupPlan& P = gp[gi];
+        P.fold.assign(G.op_count, 0);
+        int seen = 0;
+        for (int o = o0; o < o1; ++o) {
+            const TileOp& t = ops[p.op_first + o];
+            if (foldable(t) && !(seen & (1 << t.sp0))) P.fold[o - o0] = 1;
+            else seen |= touch(t);
+        }
+        seen = 0;
+        for (int o = o1 - 1; o >= o0; --o) {
+            const TileOp& t = ops[p.op_first + o];
+            if (P.fold[o - o0]) continue;
+            if (foldable(t) && !(seen & (1 << t.sp0))) P.fold[o - o0] = 2;
+            else seen |= touch(t);
+        }
+        // pending local diagonals sink past non-diagonal ops on other bits
+        std::vector<int> pend;
+        int pend_dep = 0;
+        auto flush = [&] {
+            if (pend.size() >= 2 && (int)runs.size() < kMaxRuns) {
+                P.items.emplace_back(1, (int)runs.size());
+                runs.push_back(pend);
+            } else {
+                for (int o : pend) P.items.emplace_back(0, o);
+            }
+            pend.clear();
+            pend_dep = 0;
+        };
+        for (int o = o0; o < o1; ++o) {
+            if (P.fold[o - o0]) continue;
+            const TileOp& t = ops[p.op_first + o];
+            if (local_diag(t)) {
+                pend.push_back(o);
+                pend_dep |= touch(t);
+            } else if (t.kind == kTDiag) {
+                P.items.emplace_back(0, o);  // diagonals commute with each other
+            } else {
+                int sup = 1 << t.sp0;
+                if (t.kind == kTSwap2 || t.kind == kTQuad) sup |= 1 << t.sp1;
+                if (sup & pend_dep) flush();
+                P.items.emplace_back(0, o);
+            }
+        }
+        flush();
+    }
+
     std::ostringstream s;
     s << "#include \"lq_tile_ops.h\"\nusing namespace lq;\n";
     if (dtype == LQ_C128)
         s << "typedef double T;\ntypedef double V __attribute__((ext_vector_type(2)));\n";
     else
         s << "typedef float T;\ntypedef float V __attribute__((ext_vector_type(2)));\n";
-    s << "constexpr int K = " << K << ", HB = " << p.hb << ", NT = " << jit_threads(K) << ";\n";
+    s << "constexpr int K = " << K << ", HB = " << p.hb << ", NT = " << NT << ";\n";
     s << "extern \"C\" __global__ void __launch_bounds__(NT, 2) lq_pass(V* __restrict__ psi, TilePass p, "
          "const TileOp* __restrict__ ops, uint64_t ninst, uint64_t inst_stride, uint64_t ops_stride) {\n";
     s << "  __shared__ __attribute__((aligned(16))) V tile[1u << K];\n";
     s << "  __shared__ uint64_t hoff[1u << HB];\n";
+    if (!runs.empty()) s << "  __shared__ __attribute__((aligned(16))) V ptab[" << runs.size() * 16 << "];\n";
     s << "  j_pass_begin<K, HB, NT>(hoff, p);\n";
     s << "  const uint64_t total = ninst * p.ntiles;\n";
     s << "  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {\n";
@@ -76,45 +163,31 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
     s << "    V* st = psi + (uint64_t)inst * inst_stride;\n";
     s << "    const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;\n";
+    if (!runs.empty()) {
+        // phase tables: run r by the sixteen threads of slice r mod NT/16;
+        // the tile load's barrier publishes them
+        s << "    {\n      const int ts = (int)(threadIdx.x & 15u), tg = (int)(threadIdx.x >> 4);\n";
+        for (size_t r = 0; r < runs.size(); ++r) {
+            s << "      if (tg == " << (int)(r % (size_t)(NT / 16)) << ") {\n        double pr = 1.0, pi = 0.0;\n";
+            for (int o : runs[r]) {
+                const TileOp& t = ops[p.op_first + o];
+                s << "        j_phase_mul<" << t.s0p << ", " << t.s1p << ", " << t.cin << ">(pr, pi, ts, op[" << o
+                  << "]);\n";
+            }
+            s << "        ptab[" << r * 16 << " + ts] = V{(T)pr, (T)pi};\n      }\n";
+        }
+        s << "    }\n";
+    }
     s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
     for (int gi = 0; gi < p.group_count; ++gi) {
         const TileGroup& G = groups[p.group_first + gi];
-        // X gates with their controls outside the subcube and nothing but
-        // other such gates on their bit before (after) them in the group are
-        // done by flipping the gather (scatter) slots of the threads whose
-        // controls are set.
+        const GroupPlan& P = gp[gi];
         const int o0 = G.op_first, o1 = G.op_first + G.op_count;
-        std::vector<int> fold(G.op_count, 0);  // 1: gather, 2: scatter
-        auto touch = [&](const TileOp& t) -> int {
-            int m = t.cin;
-            if (t.kind == kTDiag) {
-                if (t.s0p >= 0) m |= 1 << t.s0p;
-                if (t.s1p >= 0) m |= 1 << t.s1p;
-            } else {
-                m |= 1 << t.sp0;
-                if (t.kind == kTSwap2 || t.kind == kTQuad) m |= 1 << t.sp1;
-            }
-            return m;
-        };
-        auto foldable = [&](const TileOp& t) { return t.kind == kTPairSwap && t.cin == 0 && t.cout != 0; };
-        int seen = 0;
-        for (int o = o0; o < o1; ++o) {
-            const TileOp& t = ops[p.op_first + o];
-            if (foldable(t) && !(seen & (1 << t.sp0))) fold[o - o0] = 1;
-            else seen |= touch(t);
-        }
-        seen = 0;
-        for (int o = o1 - 1; o >= o0; --o) {
-            const TileOp& t = ops[p.op_first + o];
-            if (fold[o - o0]) continue;
-            if (foldable(t) && !(seen & (1 << t.sp0))) fold[o - o0] = 2;
-            else seen |= touch(t);
-        }
         auto flips = [&](int which) {
             std::ostringstream f;
             f << "[&](uint64_t gb) -> uint32_t { uint32_t f = 0u;";
             for (int o = o0; o < o1; ++o) {
-                if (fold[o - o0] != which) continue;
+                if (P.fold[o - o0] != which) continue;
                 const TileOp& t = ops[p.op_first + o];
                 f << " if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) f ^= "
                   << (1u << G.spos[t.sp0]) << "u;";
@@ -124,10 +197,16 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         };
         s << "    j_group<K, HB, NT, " << G.spos[0] << ", " << G.spos[1] << ", " << G.spos[2] << ", " << G.spos[3]
           << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
-        for (int o = G.op_first; o < G.op_first + G.op_count; ++o) {
-            const TileOp& t = ops[p.op_first + o];
-            if (fold[o - o0]) continue;
+        for (const auto& [what, idx] : P.items) {
             s << "      ";
+            if (what == 1) {
+                int dep = 0;
+                for (int o : runs[idx]) dep |= touch(ops[p.op_first + o]);
+                s << "j_diag_tab<" << dep << ", T>(xr, xi, ptab + " << idx * 16 << ");\n";
+                continue;
+            }
+            const int o = idx;
+            const TileOp& t = ops[p.op_first + o];
             if (t.cout) s << "if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) ";
             switch (t.kind) {
                 case kTDiag: {

@@ -281,6 +281,34 @@ __device__ __forceinline__ void j_diag(T* xr, T* xi, const TileOp& op, int tb0, 
     }
 }
 
+// Merged runs of diagonal ops: (pr, pi) *= the op's phase at subcube slot
+// ts (1 where the op's in-subcube controls CINV are not all set), and the
+// run applied from its 16-entry table (entries agree on the bits outside
+// DEP, the subcube positions the run depends on).
+template <int S0, int S1, int CINV>
+__device__ __forceinline__ void j_phase_mul(double& pr, double& pi, int ts, const TileOp& op) {
+    if ((ts & CINV) != CINV) return;
+    const int b0 = S0 >= 0 ? ((ts >> (S0 >= 0 ? S0 : 0)) & 1) : 0;
+    const int b1 = S1 >= 0 ? ((ts >> (S1 >= 0 ? S1 : 0)) & 1) : 0;
+    const int k = b0 | (b1 << 1);
+    const double dr = op.m[2 * k], di = op.m[2 * k + 1];
+    const double nr = pr * dr - pi * di;
+    pi = pr * di + pi * dr;
+    pr = nr;
+}
+
+template <int DEP, typename T, typename V>
+__device__ __forceinline__ void j_diag_tab(T* xr, T* xi, const V* tab) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const V ph = tab[c & DEP];
+        const T yr = ph.x * xr[c] - ph.y * xi[c];
+        const T yi = ph.x * xi[c] + ph.y * xr[c];
+        xr[c] = yr;
+        xi[c] = yi;
+    }
+}
+
 // ---- pass skeleton of the specialised kernels --------------------------------
 // Same tile walk as the generic kernel, with the tile geometry (K tile bits,
 // HB of them high) fixed at compile time.
