@@ -157,8 +157,12 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "  __shared__ uint64_t hoff[1u << HB];\n";
     if (!runs.empty()) s << "  __shared__ __attribute__((aligned(16))) V ptab[" << runs.size() * 16 << "];\n";
     s << "  j_pass_begin<K, HB, NT>(hoff, p);\n";
+    // each block walks a contiguous range of tiles: consecutive tiles share
+    // an instance, so its op table stays in the scalar cache
     s << "  const uint64_t total = ninst * p.ntiles;\n";
-    s << "  for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {\n";
+    s << "  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;\n";
+    s << "  const uint64_t w0 = (uint64_t)blockIdx.x * per, w1 = w0 + per < total ? w0 + per : total;\n";
+    s << "  for (uint64_t w = w0; w < w1; ++w) {\n";
     s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));\n";
     s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
     s << "    V* st = psi + (uint64_t)inst * inst_stride;\n";
