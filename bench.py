@@ -30,7 +30,7 @@ def traffic_from_profiles(mode):
     """HBM bytes per launch of the dominant kernel, measured with rocprofv3
     --pmc (FETCH_SIZE and WRITE_SIZE runs, scaled by the same counters on a
     copy kernel of known size) and recorded in profiles/; None if absent."""
-    name = {"per_gate": "traffic_k_pair_rx_30q_c128.txt", "fused": "traffic_k_tileg_30q_c128.txt"}[mode]
+    name = {"per_gate": "traffic_k_pair_rx_30q_c128.txt", "fused": "traffic_lq_pass_30q_c128.txt"}[mode]
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -232,7 +232,7 @@ def uneven_batch_bench(lq, reps=2):
     return res
 
 
-def warmstart_bench(lq, count=4):
+def warmstart_bench(lq, count=4, tile=None):
     """Learned warm-start VQE at 20 qubits: XYZ rings with J ~ U[-1,1]^3,
     the L=8 RY/RZ + CZ-ring ansatz (P=320), network 3-256-256-320 trained on
     the GPU (bf16 MFMA, f32 accumulate) on the committed converged solutions
@@ -268,13 +268,18 @@ def warmstart_bench(lq, count=4):
                                          learning_rate=ev.get("lr") if ev else None)
     t_train = time.perf_counter() - t0
     Jte = warmstart.couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][:count]
-    # the recorded evaluation ran the generic tile kernel, and so does its
-    # live re-check (equal iteration counts show the record reproduces)
+    # the recorded evaluation ran the generic tile kernel at the default tile
+    # geometry, and so does its live re-check (equal iteration counts show
+    # the record reproduces; the runs are chaotic, so any other rounding
+    # moves them)
     lq.jit.set_jit(lq.jit.OFF)
+    lq._lib.call("lq_set_fused_config", 0, 0, 0)
     try:
         rows = warmstart.evaluate(ans, ws, Jte, theta0, lr_p, lr_w, cap, tol)
     finally:
         lq.jit.set_jit(lq.jit.REPEAT)
+        if tile:
+            lq._lib.call("lq_set_fused_config", *tile)
     # per-iteration wall time of each method at the first held-out coupling
     obs = warmstart.hamiltonians(n, Jte[:1])[0]
     seq, bat, adj = lq.ParameterShift(batched=False), lq.ParameterShift(batched=True), lq.AdjointGradient()
@@ -595,7 +600,11 @@ def main():
         print("bench: uneven batch workload", file=sys.stderr, flush=True)
         extra["uneven_batch"] = uneven_batch_bench(lq)
         print("bench: warm-start VQE workload", file=sys.stderr, flush=True)
-        extra["warmstart_vqe_20q"] = warmstart_bench(lq)
+        tile = None
+        if tune:
+            tc = tune["tile_choice"]
+            tile = ((tc["tile_bits"], 0, tc["low_bits"]) if dtype == lq.C128 else (0, tc["tile_bits"], tc["low_bits"]))
+        extra["warmstart_vqe_20q"] = warmstart_bench(lq, tile=tile)
     cpu = None
     if not a.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)

@@ -639,8 +639,12 @@ __global__ void __launch_bounds__(tile_threads(K), 2) k_tileg(V* __restrict__ ps
     }
     __syncthreads();
     const TileGroup* grp = groups + p.group_first;
+    // a contiguous range of tiles per block: consecutive tiles share an
+    // instance, so its op table stays in the scalar cache
     const uint64_t total = ninst * p.ntiles;
-    for (uint64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint64_t w0 = (uint64_t)blockIdx.x * per, w1 = w0 + per < total ? w0 + per : total;
+    for (uint64_t w = w0; w < w1; ++w) {
         // wave-uniform by construction; readfirstlane lets the compiler see it,
         // so op parameters are fetched with scalar loads
         const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));

@@ -430,6 +430,24 @@ def cmd_merge(a):
     print("merged %d parts, %d couplings -> %s" % (len(parts), len(rows), a.out))
 
 
+def cmd_time_gradient(a):
+    """Wall time of single gradients of the benchmark ansatz (one XYZ ring
+    coupling, the plain initialisation), one line per call."""
+    import time
+    from .optimization import ParameterShift, AdjointGradient, ry_rz_cz_ring_ansatz, heisenberg_ring
+    from ._lib import call
+    ans = ry_rz_cz_ring_ansatz(a.qubits, a.layers)
+    obs = heisenberg_ring(a.qubits, 0.3, -0.7, 0.5)
+    theta = plain_init(ans.num_parameters(), 5)
+    method = {"batched": ParameterShift(batched=True), "sequential": ParameterShift(batched=False),
+              "adjoint": AdjointGradient()}[a.method]
+    for i in range(a.calls):
+        t = time.perf_counter()
+        e, _ = method.compute_gradient(ans, obs, theta, energy=True)
+        call("lq_synchronize")
+        print("call %d: %.4f s (energy %.12f)" % (i, time.perf_counter() - t, e), flush=True)
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="python -m logosq_amd.warmstart")
@@ -455,8 +473,14 @@ def main(argv=None):
     m = sub.add_parser("merge", help="join evaluation parts into one file")
     m.add_argument("--out", required=True)
     m.add_argument("parts", nargs="+")
+    t = sub.add_parser("time-gradient", help="wall time of gradients of the benchmark ansatz")
+    t.add_argument("--qubits", type=int, default=20)
+    t.add_argument("--layers", type=int, default=8)
+    t.add_argument("--method", choices=["batched", "sequential", "adjoint"], default="batched")
+    t.add_argument("--calls", type=int, default=3)
     a = ap.parse_args(argv)
-    {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge}[a.cmd](a)
+    {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge,
+     "time-gradient": cmd_time_gradient}[a.cmd](a)
 
 
 if __name__ == "__main__":
