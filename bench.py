@@ -449,8 +449,10 @@ def run_circuit(lq, st, arr, count, mode, steps, warmup, label):
 def expectation_bench(lq, st, n, c, reps=3):
     """Pauli-sum expectation on the circuit's output state: the periodic XYZ
     ring, 3n terms with J = (0.7, -0.4, 0.9).  The XX and YY terms of a bond
-    share one flip mask and every ZZ term is diagonal, so the sum is n + 1
-    groups, each one read of the state (c*N bytes).  HIP-event time of the
+    share one flip mask and every ZZ term is diagonal: n + 1 groups.  The
+    flip groups are evaluated from LDS tiles, a few layouts each reading the
+    state once; `reads` counts the state reads (layouts + groups read alone +
+    the diagonal group), `bytes` = reads * c * N.  HIP-event time of the
     whole lq_expect_pauli_sum call (fastest of `reps`)."""
     import numpy as np
     codes, coeffs = lq.algorithms.heisenberg_codes(n, 0.7, -0.4, 0.9)
@@ -458,13 +460,27 @@ def expectation_bench(lq, st, n, c, reps=3):
     e = ctypes.c_double()
     cp = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
     fp = coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    dtype = 0 if c == 16 else 1
+    lay, pair, diag = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lq._lib.call("lq_expect_plan", n, dtype, cp, fp, len(coeffs), 0, ctypes.byref(lay), ctypes.byref(pair),
+                 ctypes.byref(diag))
+    reads = lay.value + pair.value + diag.value
     fn = lambda: lq._lib.call("lq_expect_pauli_sum", st.handle, cp, fp, len(coeffs), 0, ctypes.byref(e))
     fn()
     t = min(event_time(lq, fn) for _ in range(reps))
-    gbs = groups * c * (1 << n) / t / 1e9
-    return {"workload": "periodic XYZ ring, %d terms, %d flip groups" % (len(coeffs), groups),
-            "ms": round(t * 1e3, 3), "bytes": groups * c * (1 << n), "GBps": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "energy": e.value}
+    lq._lib.call("lq_set_expect_tiles", 0)  # one read per group, for comparison
+    try:
+        fn()
+        t_groups = min(event_time(lq, fn) for _ in range(reps))
+    finally:
+        lq._lib.call("lq_set_expect_tiles", 1)
+    gbs = reads * c * (1 << n) / t / 1e9
+    return {"workload": "periodic XYZ ring, %d terms, %d groups (%d flip groups in %d tile layouts, %d read "
+                        "alone, %d diagonal)" % (len(coeffs), groups, groups - diag.value - pair.value, lay.value,
+                                                 pair.value, diag.value),
+            "ms": round(t * 1e3, 3), "reads": reads, "bytes": reads * c * (1 << n), "GBps": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "ms_one_read_per_group": round(t_groups * 1e3, 3),
+            "energy": e.value}
 
 
 def c64_error(lq, n, depth, seed):

@@ -250,6 +250,21 @@ int lq_noise_depolarizing(lq_state* st, const int* qubits, int count, double p, 
  * qubit q.  convention is LQ_PAULI_REFERENCE or LQ_PAULI_PHYSICAL. */
 int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
                         int convention, double* out);
+/* How flip groups (terms sharing their X/Y bits) are evaluated (no
+ * reference counterpart): LQ_EXPECT_TILES_AUTO (default) reads a state over
+ * 64 MiB once per tile layout, evaluating every flip group whose bits lie in
+ * the layout's 2^11 (complex128) / 2^12 (complex64) amplitude tiles from
+ * LDS; smaller states, and LQ_EXPECT_TILES_OFF, read the state once per
+ * group; LQ_EXPECT_TILES_ON uses tiles at every size from 8 qubits.  Also
+ * used by the batched gradients.  Results agree to rounding. */
+#define LQ_EXPECT_TILES_OFF 0
+#define LQ_EXPECT_TILES_AUTO 1
+#define LQ_EXPECT_TILES_ON 2
+int lq_set_expect_tiles(int mode);
+/* The reads of the state an expectation of this sum makes in the current
+ * mode: tile layouts, flip groups read on their own, diagonal groups. */
+int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
+                   int convention, int* tile_layouts, int* pair_groups, int* diag_groups);
 
 /* ---- Gradients (src/optimization/gradient.rs) --------------------------- */
 /* ParameterShift::compute_gradient (gradient.rs:45-78) for an ansatz given as

@@ -76,6 +76,8 @@ SIGNATURES = {
     "lq_set_batch_graphs": [c_int],
     "lq_batch_graph_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
     "lq_set_jit": [c_int],
+    "lq_set_expect_tiles": [c_int],
+    "lq_expect_plan": [c_int, c_int, P_u8, P_double, c_int, c_int, P_int, P_int, P_int],
     "lq_jit_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                      ctypes.POINTER(ctypes.c_longlong), P_double],
     "lq_jit_compile_plan": [c_int, c_int, P_gate, c_int, P_int],

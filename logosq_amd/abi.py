@@ -29,6 +29,7 @@ GATE_FLAG_RENORMALIZE = 1
 
 PAULI_I, PAULI_X, PAULI_Y, PAULI_Z = 0, 1, 2, 3
 PAULI_REFERENCE, PAULI_PHYSICAL = 0, 1
+EXPECT_TILES_OFF, EXPECT_TILES_AUTO, EXPECT_TILES_ON = 0, 1, 2
 
 EXEC_PER_GATE, EXEC_FUSED = 0, 1
 GRAD_SEQUENTIAL, GRAD_BATCHED, GRAD_ADJOINT = 0, 1, 2

@@ -155,14 +155,18 @@ __global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict_
     if (threadIdx.x == 0) partial[b] = t;
 }
 
-// energies[inst] = sum of the partials of the instance's blocks, in order.
+// energies[inst] = sum of the partials of the instance's blocks, then of
+// its LDS-tile partials (tpart[layout][inst][gx]), in a fixed order.
 __global__ void __launch_bounds__(256) k_sum_ranges(const double* __restrict__ partial,
                                                    const int* __restrict__ first, const int* __restrict__ count,
+                                                   const double* __restrict__ tpart, int nl, int gx, int ninst,
                                                    double* __restrict__ out) {
     __shared__ double sh[256];
     const int inst = blockIdx.x;
     double s = 0.0;
     for (int i = threadIdx.x; i < count[inst]; i += 256) s += partial[first[inst] + i];
+    for (int q = threadIdx.x; q < nl * gx; q += 256)
+        s += tpart[((size_t)(q / gx) * ninst + inst) * gx + q % gx];
     double t = bsum<256>(s, sh);
     if (threadIdx.x == 0) out[inst] = t;
 }
@@ -200,6 +204,13 @@ struct BatchJob {
     size_t nops = 0, goff = 0, plan_bytes = 0;
     int nitems = 0, blocks = 0;
     size_t roff = 0, items_bytes = 0, terms_bytes = 0;
+    // flip groups evaluated from LDS tiles (when every instance shares one
+    // sum): layouts, their tables in the items buffer, partials after the
+    // items' partials
+    std::vector<ExpLayout> lays;
+    std::vector<size_t> lay_off;
+    int tile_gx = 0;
+    size_t tpart_off = 0;
 };
 
 // Pinned space for the op tables of ninst instances (filled by the caller),
@@ -228,10 +239,23 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     std::vector<ExpItem> items;
     std::vector<int> first(j.ninst), count(j.ninst);
     int blocks = 0;
+    j.lays.clear();
+    j.lay_off.clear();
+    j.tile_gx = 0;
+    std::vector<char> tiled;
+    if (sums.size() == 1) {
+        std::vector<int> rest;
+        plan_expect_tiles(j.n, j.dtype, sums[0], j.lays, rest);
+        tiled.assign(sums[0].groups.size(), 0);
+        for (size_t g = 0; g < tiled.size(); ++g) tiled[g] = sums[0].groups[g].x != 0;
+        for (int g : rest) tiled[g] = 0;
+        if (!j.lays.empty()) j.tile_gx = (int)expect_tile_blocks(j.lays[0].geo, (uint64_t)j.ninst);
+    }
     for (int i = 0; i < j.ninst; ++i) {
         const PauliSum& ps = sums[obs_of[i]];
         first[i] = blocks;
         for (const PauliGroup& g : ps.groups) {
+            if (!tiled.empty() && tiled[&g - ps.groups.data()]) continue;
             ExpItem it;
             std::memset(&it, 0, sizeof(it));
             it.state_off = (uint64_t)i * j.dim;
@@ -252,12 +276,19 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     j.blocks = blocks;
     const size_t ib = items.size() * sizeof(ExpItem);
     const size_t rb = (size_t)j.ninst * 2 * sizeof(int);
-    j.roff = (ib + 255) & ~(size_t)255;
-    j.items_bytes = j.roff + rb;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    j.roff = al(ib);
+    j.items_bytes = al(j.roff + rb);
+    for (const ExpLayout& l : j.lays) {
+        j.lay_off.push_back(j.items_bytes);
+        j.items_bytes += al(l.groups.size() * sizeof(ExpTileGroup)) + al(l.terms.size() * sizeof(ExpTileTerm));
+    }
     j.terms_bytes = std::max<size_t>(nterms, 1) * sizeof(PauliTermDev);
+    j.tpart_off = (size_t)std::max(blocks, 1);
+    const size_t tparts = j.lays.size() * (size_t)j.ninst * (size_t)j.tile_gx;
     int rc = g_items.get(j.items_bytes + 256);
     if (!rc) rc = g_terms.get(j.terms_bytes);
-    if (!rc) rc = g_part.get((size_t)std::max(blocks, 1) * sizeof(double));
+    if (!rc) rc = g_part.get((j.tpart_off + tparts + 1) * sizeof(double));
     if (!rc) rc = g_out.get((size_t)j.ninst * sizeof(double));
     if (!rc) rc = g_hitems.get(j.items_bytes + 256);
     if (!rc) rc = g_hterms.get(j.terms_bytes);
@@ -267,6 +298,12 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     if (ib) std::memcpy(hi, items.data(), ib);
     std::memcpy(hi + j.roff, first.data(), j.ninst * sizeof(int));
     std::memcpy(hi + j.roff + j.ninst * sizeof(int), count.data(), j.ninst * sizeof(int));
+    for (size_t l = 0; l < j.lays.size(); ++l) {
+        const ExpLayout& lay = j.lays[l];
+        std::memcpy(hi + j.lay_off[l], lay.groups.data(), lay.groups.size() * sizeof(ExpTileGroup));
+        std::memcpy(hi + j.lay_off[l] + al(lay.groups.size() * sizeof(ExpTileGroup)), lay.terms.data(),
+                    lay.terms.size() * sizeof(ExpTileTerm));
+    }
     PauliTermDev* ht = (PauliTermDev*)g_hterms.p;
     for (size_t k = 0; k < sums.size(); ++k)
         if (!sums[k].terms.empty())
@@ -306,22 +343,36 @@ static int job_launch(const BatchJob& j, void* psi) {
         rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)j.ninst, j.dim, j.nops, s.jit);
         if (rc) return rc;
     }
-    if (j.blocks > 0) {
+    if (j.blocks > 0 || !j.lays.empty()) {
         LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, g_hitems.p, j.items_bytes, hipMemcpyHostToDevice, stream()));
         LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, g_hterms.p, j.terms_bytes, hipMemcpyHostToDevice, stream()));
         const int* d_first = (const int*)((char*)g_items.p + j.roff);
         const int* d_count = d_first + j.ninst;
-        if (j.dtype == LQ_C128)
-            hipLaunchKernelGGL(k_expect_items<double2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
-                               (const double2*)psi, (const ExpItem*)g_items.p, j.nitems,
-                               (const PauliTermDev*)g_terms.p, (double*)g_part.p);
-        else
-            hipLaunchKernelGGL(k_expect_items<float2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
-                               (const float2*)psi, (const ExpItem*)g_items.p, j.nitems,
-                               (const PauliTermDev*)g_terms.p, (double*)g_part.p);
-        LQ_LAUNCH_CHECK();
+        if (j.blocks > 0) {
+            if (j.dtype == LQ_C128)
+                hipLaunchKernelGGL(k_expect_items<double2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
+                                   (const double2*)psi, (const ExpItem*)g_items.p, j.nitems,
+                                   (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+            else
+                hipLaunchKernelGGL(k_expect_items<float2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
+                                   (const float2*)psi, (const ExpItem*)g_items.p, j.nitems,
+                                   (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+            LQ_LAUNCH_CHECK();
+        }
+        double* d_tpart = (double*)g_part.p + j.tpart_off;
+        auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        for (size_t l = 0; l < j.lays.size(); ++l) {
+            const ExpLayout& lay = j.lays[l];
+            const char* dp = (const char*)g_items.p + j.lay_off[l];
+            int rc = launch_expect_tile(psi, j.dtype, lay.geo, (uint64_t)j.ninst, j.dim, (const ExpTileGroup*)dp,
+                                        (int)lay.groups.size(),
+                                        (const ExpTileTerm*)(dp + al(lay.groups.size() * sizeof(ExpTileGroup))),
+                                        (unsigned)j.tile_gx, d_tpart + l * (size_t)j.ninst * j.tile_gx);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_sum_ranges, dim3(j.ninst), dim3(256), 0, stream(), (const double*)g_part.p, d_first,
-                           d_count, (double*)g_out.p);
+                           d_count, (const double*)d_tpart, (int)j.lays.size(), j.tile_gx, j.ninst,
+                           (double*)g_out.p);
         LQ_LAUNCH_CHECK();
         LQ_HIP_CHECK(hipMemcpyAsync(g_hout.p, g_out.p, j.ninst * sizeof(double), hipMemcpyDeviceToHost, stream()));
     }
@@ -345,6 +396,7 @@ static long g_graph_clock = 0;
 static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
     std::vector<uint64_t> k = {(uint64_t)j.n, (uint64_t)j.dtype, (uint64_t)j.ninst, j.nops, j.goff, j.plan_bytes,
                                (uint64_t)j.nitems, (uint64_t)j.blocks, j.roff, j.items_bytes, j.terms_bytes,
+                               (uint64_t)j.lays.size(), (uint64_t)j.tile_gx, j.tpart_off,
                                (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p, (uint64_t)g_items.p,
                                (uint64_t)g_hitems.p, (uint64_t)g_terms.p, (uint64_t)g_hterms.p, (uint64_t)g_part.p,
                                (uint64_t)g_out.p, (uint64_t)g_hout.p, j.plan->steps.size()};
@@ -358,6 +410,11 @@ static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
         k.push_back(((uint64_t)(uint32_t)p.rich << 32));
         k.push_back(p.ntiles);
         k.push_back((uint64_t)(uintptr_t)s.jit);
+    }
+    for (const ExpLayout& l : j.lays) {
+        k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
+        for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
+        k.push_back(((uint64_t)l.groups.size() << 32) | l.terms.size());
     }
     return k;
 }
@@ -422,7 +479,7 @@ static int job_run(const BatchJob& j, void* psi, double* energies) {
     }
     if (rc) return rc;
     LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-    if (j.blocks > 0) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
+    if (j.blocks > 0 || !j.lays.empty()) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
     else
         for (int i = 0; i < j.ninst; ++i) energies[i] = 0.0;
     return LQ_OK;

@@ -15,8 +15,10 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <type_traits>
 
 #include "lq_host.h"
+#include "lq_tile_ops.h"
 
 namespace lq {
 
@@ -155,6 +157,257 @@ __global__ void __launch_bounds__(kExpThreads) k_expect_flip(const V* __restrict
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ---- flip groups from LDS tiles ------------------------------------------
+// A tile of 2^K amplitudes (the L lowest bits and hb chosen high bits, as in
+// the fused engine's passes) is read into LDS once, and every flip group
+// whose mask lies inside the tile's bits is evaluated from there: one read
+// of the state serves all of them.  For a pair (i, j = i ^ x) and a term t
+// with sign s_t(i) = (-1)^{popc(i & zy_t)}, the two rows' contributions add
+// up to s_t(i) * g_t * Re(conj(psi_i) psi_j) when popc(x & zy_t) is even
+// (g_t = 2 cr_t) and s_t(i) * g_t * Im(conj(psi_i) psi_j) when it is odd
+// (g_t = -2 ci_t).  popc(i & zy_t) splits into the tile base's bits outside
+// the tile (zyo, uniform per tile) and the local index's bits (zyl).
+// 32 KiB tiles (2^11 complex128, 2^12 complex64): four blocks per CU hide
+// the LDS latency better than two blocks of 64 KiB tiles, and the smaller
+// tiles still cover the usual local Hamiltonians in a few layouts.
+template <int K, typename V>
+__global__ void __launch_bounds__(kExpThreads, ((sizeof(V) << K) <= 32768 ? 4 : 2))
+    k_expect_tile(const V* __restrict__ psi, ExpGeom geo, uint64_t inst_stride,
+                  const ExpTileGroup* __restrict__ groups, int ngroups, const ExpTileTerm* __restrict__ terms,
+                  double* __restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr uint32_t TS = 1u << K;
+    constexpr int EPT = TS >= (uint32_t)kExpThreads ? (int)(TS / kExpThreads) : 1;
+    V* tile = reinterpret_cast<V*>(smem);
+    uint64_t* hoff = reinterpret_cast<uint64_t*>(smem + TS * sizeof(V));
+    double* sh = reinterpret_cast<double*>(smem);  // reused for the final sum
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lowmask = (1u << geo.L) - 1u;
+    for (uint32_t h = tid; h < (1u << geo.hb); h += kExpThreads) {
+        uint64_t o = 0;
+        for (int j = 0; j < geo.hb; ++j)
+            if ((h >> j) & 1u) o |= 1ull << geo.hpos[j];
+        hoff[h] = o;
+    }
+    __syncthreads();
+    const V* s = psi + (uint64_t)blockIdx.y * inst_stride;
+    const uint64_t per = (geo.ntiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * per, t1 = t0 + per < geo.ntiles ? t0 + per : geo.ntiles;
+    double acc = 0.0;
+    for (uint64_t t = t0; t < t1; ++t) {
+        uint64_t base = t << geo.L;
+        for (int j = 0; j < geo.hb; ++j) base = insert_zero(base, geo.hpos[j]);
+        {
+            V r[EPT];
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) {
+                const uint32_t e = tid + (uint32_t)j * kExpThreads;
+                if (e < TS) r[j] = s[base | (e & lowmask) | hoff[e >> geo.L]];
+            }
+#pragma unroll
+            for (int j = 0; j < EPT; ++j) {
+                const uint32_t e = tid + (uint32_t)j * kExpThreads;
+                if (e < TS) tile[swz(e)] = r[j];
+            }
+        }
+        __syncthreads();
+        constexpr int PPT = TS / 2 >= (uint32_t)kExpThreads ? (int)(TS / 2 / kExpThreads) : 1;
+        for (int g = 0; g < ngroups; ++g) {
+            const ExpTileGroup G = groups[g];
+            // terms four at a time, their tile-uniform sign folded into the
+            // weights (gr for the real part, gi for the imaginary part)
+            for (int q0 = G.first; q0 < G.first + G.count; q0 += 4) {
+                const int nq = G.first + G.count - q0 < 4 ? G.first + G.count - q0 : 4;
+                uint32_t zl[4];
+                double gr[4], gi[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    zl[j] = 0u;
+                    gr[j] = gi[j] = 0.0;
+                    if (j < nq) {
+                        const ExpTileTerm T = terms[q0 + j];
+                        const double gs = (__popcll(base & T.zyo) & 1) ? -T.g : T.g;
+                        zl[j] = T.zyl;
+                        gr[j] = T.im ? 0.0 : gs;
+                        gi[j] = T.im ? gs : 0.0;
+                    }
+                }
+                auto pairs = [&](auto nqc) {
+                    constexpr int NQ = decltype(nqc)::value;
+#pragma unroll
+                    for (int u = 0; u < PPT; ++u) {
+                        const uint32_t k = tid + (uint32_t)u * kExpThreads;
+                        if (k >= TS / 2) break;
+                        const uint32_t e = ins0(k, G.hl);
+                        const V a = tile[swz(e)], b = tile[swz(e ^ G.xl)];
+                        const double zr = (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+                        const double zi = (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+                        double wr = 0.0, wi = 0.0;
+#pragma unroll
+                        for (int j = 0; j < NQ; ++j) {
+                            const double sg = (__popc(e & zl[j]) & 1) ? -1.0 : 1.0;
+                            wr += sg * gr[j];
+                            wi += sg * gi[j];
+                        }
+                        acc += zr * wr + zi * wi;
+                    }
+                };
+                switch (nq) {
+                    case 1: pairs(std::integral_constant<int, 1>{}); break;
+                    case 2: pairs(std::integral_constant<int, 2>{}); break;
+                    case 3: pairs(std::integral_constant<int, 3>{}); break;
+                    default: pairs(std::integral_constant<int, 4>{}); break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double r = block_sum_e<kExpThreads>(acc, sh);
+    if (tid == 0) partial[(uint64_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+}
+
+static int exp_tile_bits(int dtype) { return dtype == LQ_C128 ? 11 : 12; }
+
+// LQ_EXPECT_TILES_AUTO: tiles for states over 64 MiB.  Smaller states stay
+// in the Infinity Cache between the per-group reads, which are then cheaper
+// than the tile kernel's LDS work.
+static int g_exp_tiles = LQ_EXPECT_TILES_AUTO;
+
+bool expect_tiles_wanted(int n, int dtype) {
+    if (g_exp_tiles == LQ_EXPECT_TILES_OFF) return false;
+    if (g_exp_tiles == LQ_EXPECT_TILES_ON) return true;
+    const uint64_t bytes = (dtype == LQ_C128 ? 16ull : 8ull) << n;
+    return bytes > (64ull << 20);
+}
+
+void plan_expect_tiles(int n, int dtype, const PauliSum& ps, std::vector<ExpLayout>& layouts,
+                       std::vector<int>& rest) {
+    layouts.clear();
+    rest.clear();
+    const int K = std::min(exp_tile_bits(dtype), n);
+    std::vector<int> flips;
+    for (int g = 0; g < (int)ps.groups.size(); ++g)
+        if (ps.groups[g].x != 0) flips.push_back(g);
+    if (n < 8 || !expect_tiles_wanted(n, dtype)) {  // tiny or cache-resident states: the pair kernel
+        rest = flips;
+        return;
+    }
+    const int L = std::min(5, K);
+    const int HB = K - L;
+    const uint64_t low = (1ull << L) - 1ull;
+    std::vector<int> todo;
+    for (int g : flips) {
+        if (__builtin_popcountll(ps.groups[g].x & ~low) <= HB) todo.push_back(g);
+        else rest.push_back(g);
+    }
+    while (!todo.empty()) {
+        uint64_t H = 0;
+        std::vector<char> taken(todo.size(), 0);
+        for (;;) {  // take the group needing the fewest new high bits
+            int best = -1, bestc = 64;
+            for (size_t i = 0; i < todo.size(); ++i) {
+                if (taken[i]) continue;
+                const uint64_t need = ps.groups[todo[i]].x & ~low & ~H;
+                const int c = __builtin_popcountll(need);
+                if (__builtin_popcountll(H | need) <= HB && c < bestc) {
+                    best = (int)i;
+                    bestc = c;
+                }
+            }
+            if (best < 0) break;
+            H |= ps.groups[todo[best]].x & ~low;
+            taken[best] = 1;
+        }
+        for (int b = L; b < n && __builtin_popcountll(H) < HB; ++b) H |= 1ull << b;
+        ExpLayout lay;
+        std::memset(&lay.geo, 0, sizeof(lay.geo));
+        lay.geo.L = L;
+        for (int b = L; b < n; ++b)
+            if (H & (1ull << b)) lay.geo.hpos[lay.geo.hb++] = b;
+        lay.geo.ntiles = 1ull << (n - K);
+        const uint64_t tilebits = low | H;
+        auto local = [&](uint64_t m) -> uint32_t {
+            uint32_t r = (uint32_t)(m & low);
+            for (int j = 0; j < lay.geo.hb; ++j)
+                if (m & (1ull << lay.geo.hpos[j])) r |= 1u << (L + j);
+            return r;
+        };
+        std::vector<int> left;
+        for (size_t i = 0; i < todo.size(); ++i) {
+            const PauliGroup& grp = ps.groups[todo[i]];
+            if ((grp.x & ~tilebits) != 0) {
+                left.push_back(todo[i]);
+                continue;
+            }
+            ExpTileGroup G;
+            G.xl = local(grp.x);
+            G.hl = 31 - __builtin_clz(G.xl);
+            G.first = (int)lay.terms.size();
+            G.count = grp.count;
+            for (int t = grp.first; t < grp.first + grp.count; ++t) {
+                const PauliTermDev& pt = ps.terms[t];
+                ExpTileTerm T;
+                T.zyo = pt.zy & ~tilebits;
+                T.zyl = local(pt.zy & tilebits);
+                T.im = __builtin_popcountll(grp.x & pt.zy) & 1;
+                T.g = T.im ? -2.0 * pt.ci : 2.0 * pt.cr;
+                lay.terms.push_back(T);
+            }
+            lay.groups.push_back(G);
+        }
+        layouts.push_back(std::move(lay));
+        todo.swap(left);
+    }
+}
+
+size_t expect_tile_lds(int dtype, const ExpGeom& geo) {
+    const size_t amp = dtype == LQ_C128 ? 16 : 8;
+    return std::max<size_t>((amp << (geo.L + geo.hb)) + (sizeof(uint64_t) << geo.hb), kExpThreads * sizeof(double));
+}
+
+int launch_expect_tile(const void* psi, int dtype, const ExpGeom& geo, uint64_t ninst, uint64_t inst_stride,
+                       const ExpTileGroup* d_groups, int ngroups, const ExpTileTerm* d_terms, unsigned gx,
+                       double* d_partial) {
+    const size_t lds = expect_tile_lds(dtype, geo);
+    const int K = geo.L + geo.hb;
+    auto go = [&](auto kc, auto vt) {
+        constexpr int KK = decltype(kc)::value;
+        using V = decltype(vt);
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void*)k_expect_tile<KK, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      96 * 1024);
+            attr = true;
+        }
+        hipLaunchKernelGGL((k_expect_tile<KK, V>), dim3(gx, (unsigned)ninst), dim3(kExpThreads), lds, stream(),
+                           (const V*)psi, geo, inst_stride, d_groups, ngroups, d_terms, d_partial);
+    };
+    auto by_k = [&](auto vt) {
+        switch (K) {
+            case 8: go(std::integral_constant<int, 8>{}, vt); break;
+            case 9: go(std::integral_constant<int, 9>{}, vt); break;
+            case 10: go(std::integral_constant<int, 10>{}, vt); break;
+            case 11: go(std::integral_constant<int, 11>{}, vt); break;
+            case 12: go(std::integral_constant<int, 12>{}, vt); break;
+            case 13: go(std::integral_constant<int, 13>{}, vt); break;
+            default: return false;
+        }
+        return true;
+    };
+    const bool ok = dtype == LQ_C128 ? by_k(double2{}) : by_k(float2{});
+    if (!ok) {
+        set_error("expectation tile of 2^%d amplitudes is not supported", K);
+        return LQ_ERR_UNSUPPORTED;
+    }
+    LQ_LAUNCH_CHECK();
+    return LQ_OK;
+}
+
+unsigned expect_tile_blocks(const ExpGeom& geo, uint64_t ninst) {
+    const uint64_t want = std::max<uint64_t>(1, 2048 / std::max<uint64_t>(ninst, 1));
+    return (unsigned)std::min<uint64_t>(geo.ntiles, want);
+}
+
 __global__ void __launch_bounds__(1024) k_expect_final(const double* __restrict__ partial, int count,
                                                       double* __restrict__ out) {
     __shared__ double sh[1024];
@@ -250,9 +503,21 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
     std::vector<PauliTermDev> dev = ps.terms;
     int diag_first = -1, diag_static = 0, diag_count = 0;
     size_t launches = 0;
+    // flip groups: from LDS tiles where they fit a layout, else pair kernels
+    std::vector<ExpLayout> lays;
+    std::vector<int> rest;
+    plan_expect_tiles(st->n, st->dtype, ps, lays, rest);
+    std::vector<char> paired(ng, 0);
+    for (int g : rest) paired[g] = 1;
+    size_t tile_parts = 0;
+    std::vector<unsigned> lay_blocks;
+    for (const ExpLayout& l : lays) {
+        lay_blocks.push_back(expect_tile_blocks(l.geo, 1));
+        tile_parts += lay_blocks.back();
+    }
     for (const PauliGroup& grp : ps.groups) {
         if (grp.x != 0) {
-            ++launches;
+            if (paired[&grp - ps.groups.data()]) ++launches;
             continue;
         }
         std::vector<PauliTermDev> st_, dy_;
@@ -265,20 +530,46 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
         diag_count = grp.count;
         launches += std::max<size_t>(1, (dy_.size() + kDiagDyn - 1) / kDiagDyn);
     }
-    size_t part_count = launches * nb;
-    char* base = (char*)scratch(term_pad + (part_count + 8) * sizeof(double));
+    // layout tables after the term table: [groups][terms] per layout
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    std::vector<size_t> lay_off;
+    size_t lay_bytes = 0;
+    for (const ExpLayout& l : lays) {
+        lay_off.push_back(lay_bytes);
+        lay_bytes += al(l.groups.size() * sizeof(ExpTileGroup)) + al(l.terms.size() * sizeof(ExpTileTerm));
+    }
+    size_t part_count = launches * nb + tile_parts;
+    char* base = (char*)scratch(term_pad + lay_bytes + (part_count + 8) * sizeof(double));
     if (!base) {
         set_error("scratch allocation failed");
         return LQ_ERR_OUT_OF_MEMORY;
     }
     PauliTermDev* d_terms = (PauliTermDev*)base;
-    double* d_part = (double*)(base + term_pad);
+    char* d_lay = base + term_pad;
+    double* d_part = (double*)(base + term_pad + lay_bytes);
     double* d_out = d_part + part_count;
-    double* h = host_staging(term_bytes / sizeof(double) + 16);
+    double* h = host_staging((term_pad + lay_bytes) / sizeof(double) + 16);
     if (!h) return LQ_ERR_OUT_OF_MEMORY;
     std::memcpy(h, dev.data(), term_bytes);
-    LQ_HIP_CHECK(hipMemcpyAsync(d_terms, h, term_bytes, hipMemcpyHostToDevice, stream()));
+    for (size_t li = 0; li < lays.size(); ++li) {
+        char* hp = (char*)h + term_pad + lay_off[li];
+        const ExpLayout& l = lays[li];
+        std::memcpy(hp, l.groups.data(), l.groups.size() * sizeof(ExpTileGroup));
+        std::memcpy(hp + al(l.groups.size() * sizeof(ExpTileGroup)), l.terms.data(),
+                    l.terms.size() * sizeof(ExpTileTerm));
+    }
+    LQ_HIP_CHECK(hipMemcpyAsync(d_terms, h, term_pad + lay_bytes, hipMemcpyHostToDevice, stream()));
     int written = 0;
+    for (size_t li = 0; li < lays.size(); ++li) {
+        const ExpLayout& l = lays[li];
+        const char* dp = d_lay + lay_off[li];
+        int rc = launch_expect_tile(st->d_amps, st->dtype, l.geo, 1, 0, (const ExpTileGroup*)dp,
+                                    (int)l.groups.size(),
+                                    (const ExpTileTerm*)(dp + al(l.groups.size() * sizeof(ExpTileGroup))),
+                                    lay_blocks[li], d_part + written);
+        if (rc) return rc;
+        written += (int)lay_blocks[li];
+    }
     for (int g = 0; g < ng; ++g) {
         const PauliGroup& grp = ps.groups[g];
         if (grp.x == 0) {
@@ -295,7 +586,7 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
                 written += nb_diag;
                 t0 += kDiagDyn;
             } while (t0 < nd_all);
-        } else {
+        } else if (paired[g]) {
             int hbit = 63 - __builtin_clzll(grp.x);
             hipLaunchKernelGGL(k_expect_flip<V>, dim3(nb_flip), dim3(kExpThreads), 0, stream(),
                                (const V*)st->d_amps, fchunks, flanes, grp.x, hbit,
@@ -328,6 +619,38 @@ int expect_pauli_sum(const lq_state* st, const PauliSum& ps, double* out) {
 }  // namespace lq
 
 using namespace lq;
+
+extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
+                              int convention, int* tile_layouts, int* pair_groups, int* diag_groups) {
+    clear_error();
+    if (num_qubits < 1 || num_qubits > kMaxQubits || num_terms < 0 || (num_terms > 0 && (!paulis || !coeffs)) ||
+        (dtype != LQ_C128 && dtype != LQ_C64)) {
+        set_error("invalid arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    PauliSum ps;
+    int rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
+    if (rc) return rc;
+    std::vector<ExpLayout> lays;
+    std::vector<int> rest;
+    plan_expect_tiles(num_qubits, dtype, ps, lays, rest);
+    int diag = 0;
+    for (const PauliGroup& g : ps.groups) diag += g.x == 0;
+    if (tile_layouts) *tile_layouts = (int)lays.size();
+    if (pair_groups) *pair_groups = (int)rest.size();
+    if (diag_groups) *diag_groups = diag;
+    return LQ_OK;
+}
+
+extern "C" int lq_set_expect_tiles(int mode) {
+    clear_error();
+    if (mode != LQ_EXPECT_TILES_OFF && mode != LQ_EXPECT_TILES_AUTO && mode != LQ_EXPECT_TILES_ON) {
+        set_error("mode must be LQ_EXPECT_TILES_OFF, LQ_EXPECT_TILES_AUTO or LQ_EXPECT_TILES_ON");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    g_exp_tiles = mode;
+    return LQ_OK;
+}
 
 extern "C" int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
                                    int convention, double* out) {

@@ -36,6 +36,42 @@ struct PauliSum {
 
 int compile_pauli_sum(int n, const uint8_t* paulis, const double* coeffs, int num_terms, int convention,
                       PauliSum* out);
+
+// Flip groups evaluated from LDS tiles: a layout is a tile geometry (the L
+// lowest bits plus hb high bits) and the flip groups whose masks lie in it.
+struct ExpGeom {
+    int32_t L, hb;
+    int32_t hpos[28];
+    uint64_t ntiles;
+};
+struct ExpTileGroup {
+    uint32_t xl;    // flip mask in tile-local bits
+    int32_t hl;     // a local bit of xl (pairs have it clear on the first row)
+    int32_t first;  // terms of the group in the layout's term table
+    int32_t count;
+};
+struct ExpTileTerm {
+    uint64_t zyo;   // sign bits outside the tile
+    uint32_t zyl;   // sign bits inside the tile, local positions
+    int32_t im;     // 1: weight of Im(conj(psi_i) psi_j), 0: of the real part
+    double g;       // 2 cr or -2 ci
+};
+struct ExpLayout {
+    ExpGeom geo;
+    std::vector<ExpTileGroup> groups;
+    std::vector<ExpTileTerm> terms;
+};
+// Cover the sum's flip groups with tile layouts; groups that fit none (or
+// all of them, for states under 8 qubits or when tiles are not wanted for
+// this size) are returned in rest (indices into ps.groups).
+bool expect_tiles_wanted(int n, int dtype);
+void plan_expect_tiles(int n, int dtype, const PauliSum& ps, std::vector<ExpLayout>& layouts, std::vector<int>& rest);
+size_t expect_tile_lds(int dtype, const ExpGeom& geo);
+unsigned expect_tile_blocks(const ExpGeom& geo, uint64_t ninst);
+// One launch per layout: partials [ninst][gx] of the layout's groups.
+int launch_expect_tile(const void* psi, int dtype, const ExpGeom& geo, uint64_t ninst, uint64_t inst_stride,
+                       const ExpTileGroup* d_groups, int ngroups, const ExpTileTerm* d_terms, unsigned gx,
+                       double* d_partial);
 // Expectation of a compiled sum on a state; result on the host.
 int expect_pauli_sum(const lq_state* st, const PauliSum& ps, double* out);
 

@@ -307,3 +307,65 @@ def test_diagonal_groups_over_32_terms(hip, n):
     for r, c in zip(codes, coeffs):
         obs.add_term(lq.PauliTerm(c, r))
     assert abs(obs.expectation(st) - oracle.pauli_sum(psi, n, codes, coeffs)) <= 1e-10
+
+
+@pytest.mark.parametrize("n", [9, 13, 16, 20])
+def test_flip_groups_from_tiles(hip, n):
+    """Local Hamiltonians (ring bonds XX, YY, ZZ, XY, YZ and 3-local
+    strings, so flip groups share tiles) plus long random strings whose flip
+    bits fit no tile: the LDS-tile flip kernel and the pair kernel together
+    against the oracle, complex128 and complex64."""
+    say("flip groups from tiles n=%d" % n)
+    rng = np.random.default_rng(5100 + n)
+    psi = bf.random_state(n, rng)
+    rows, coeffs = [], []
+    for q in range(n):
+        for a, b in ((1, 1), (2, 2), (3, 3), (1, 2), (2, 3)):
+            r = np.zeros(n, np.uint8)
+            r[q], r[(q + 1) % n] = a, b
+            rows.append(r)
+            coeffs.append(rng.normal())
+        r = np.zeros(n, np.uint8)
+        r[q], r[(q + 2) % n], r[(q + 5) % n] = 2, 1, 3
+        rows.append(r)
+        coeffs.append(rng.normal())
+    for _ in range(4):
+        rows.append(rng.integers(0, 4, n).astype(np.uint8))
+        coeffs.append(rng.normal())
+    codes, coeffs = np.array(rows), np.array(coeffs)
+    obs = lq.PauliObservable(n)
+    for r, c in zip(codes, coeffs):
+        obs.add_term(lq.PauliTerm(c, r))
+    want = oracle.pauli_sum(psi, n, codes, coeffs)
+    st = lq.State(n)
+    st.set_vector(psi)
+    s64 = lq.State(n, lq.C64)
+    s64.set_vector(psi)
+    try:
+        for mode in (abi.EXPECT_TILES_ON, abi.EXPECT_TILES_OFF, abi.EXPECT_TILES_AUTO):
+            lq._lib.call("lq_set_expect_tiles", mode)
+            assert abs(obs.expectation(st) - want) <= 1e-10, mode
+            assert abs(obs.expectation(s64) - want) <= 1e-5, mode
+    finally:
+        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
+
+
+def test_batched_gradient_flip_groups_from_tiles(hip):
+    """The batched parameter shift with its flip groups from LDS tiles (every
+    instance shares the sum) equals the per-group kernels."""
+    say("batched gradient, tiled flip groups")
+    n = 12
+    ans = lq.ry_rz_cz_ring_ansatz(n, 2)
+    obs = lq.heisenberg_ring(n, 0.6, -0.3, 0.8)
+    x = np.random.default_rng(91).uniform(0, 2 * math.pi, ans.num_parameters())
+    ps = lq.ParameterShift(batched=True)
+    try:
+        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_OFF)
+        e0, g0 = ps.compute_gradient(ans, obs, x, energy=True)
+        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_ON)
+        for _ in range(3):  # eager, captured, replayed
+            e1, g1 = ps.compute_gradient(ans, obs, x, energy=True)
+            assert abs(e1 - e0) <= 1e-12
+            assert np.max(np.abs(np.array(g1) - np.array(g0))) <= 1e-12
+    finally:
+        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
