@@ -132,23 +132,39 @@ __global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict_
             acc += p * w;
         }
     } else {
-        for (uint64_t k = (uint64_t)lb * kEThreads + threadIdx.x; k < it.work; k += stride) {
-            uint64_t i = insert_zero(k, it.h);
-            uint64_t j = i ^ it.x;
-            V a = s[i], c2 = s[j];
-            double zr = (double)a.x * (double)c2.x + (double)a.y * (double)c2.y;
-            double zi = (double)a.x * (double)c2.y - (double)a.y * (double)c2.x;
-            double wir = 0.0, wii = 0.0, wjr = 0.0, wji = 0.0;
-            for (int t = 0; t < it.term_count; ++t) {
-                PauliTermDev q = tm[t];
-                bool si = __popcll(i & q.zy) & 1;
-                bool sj = __popcll(j & q.zy) & 1;
-                wir += si ? -q.cr : q.cr;
-                wii += si ? -q.ci : q.ci;
-                wjr += sj ? -q.cr : q.cr;
-                wji += sj ? -q.ci : q.ci;
+        // Terms four at a time in registers.  For a pair (i, j = i ^ x) the
+        // two rows of term t add up to s_t(i) * 2 cr_t * Re(conj(a) b) when
+        // popc(x & zy_t) is even and s_t(i) * (-2 ci_t) * Im(conj(a) b) when
+        // it is odd, s_t(i) = (-1)^{popc(i & zy_t)}.
+        for (int q0 = 0; q0 < it.term_count; q0 += 4) {
+            const int nq = it.term_count - q0 < 4 ? it.term_count - q0 : 4;
+            uint64_t zy[4];
+            double gr[4], gi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                zy[u] = 0;
+                gr[u] = gi[u] = 0.0;
+                if (u < nq) {
+                    const PauliTermDev q = tm[q0 + u];
+                    zy[u] = q.zy;
+                    if (__popcll(it.x & q.zy) & 1) gi[u] = -2.0 * q.ci;
+                    else gr[u] = 2.0 * q.cr;
+                }
             }
-            acc += zr * wir - zi * wii + zr * wjr + zi * wji;
+            for (uint64_t k = (uint64_t)lb * kEThreads + threadIdx.x; k < it.work; k += stride) {
+                const uint64_t i = insert_zero(k, it.h);
+                const V a = s[i], c2 = s[i ^ it.x];
+                const double zr = (double)a.x * (double)c2.x + (double)a.y * (double)c2.y;
+                const double zi = (double)a.x * (double)c2.y - (double)a.y * (double)c2.x;
+                double wr = 0.0, wi = 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double sg = (__popcll(i & zy[u]) & 1) ? -1.0 : 1.0;
+                    wr += sg * gr[u];
+                    wi += sg * gi[u];
+                }
+                acc += zr * wr + zi * wi;
+            }
         }
     }
     double t = bsum<kEThreads>(acc, sh);
