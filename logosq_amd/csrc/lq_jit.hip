@@ -71,7 +71,8 @@ static constexpr int kMaxRuns = 32;
 
 struct GroupPlan {
     std::vector<int> fold;            // per op of the group: 0, 1 gather, 2 scatter
-    std::vector<std::pair<int, int>> items;  // (0, op index) or (1, run id), in order
+    std::vector<std::pair<int, int>> items;  // (0, op index), (1, run id) or (2, 0: the thread phase), in order
+    std::vector<int> uphase;          // diagonal ops on outside bits only, merged into one thread phase
 };
 
 static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype) {
@@ -92,6 +93,10 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     // diagonal and depending on subcube bits only
     auto local_diag = [&](const TileOp& t) {
         return t.kind == kTDiag && t.cout == 0 && t.osel0 < 0 && t.osel1 < 0;
+    };
+    // diagonal and depending on bits outside the subcube only
+    auto outside_diag = [&](const TileOp& t) {
+        return t.kind == kTDiag && t.cin == 0 && t.s0p < 0 && t.s1p < 0 && (t.cout != 0 || t.osel0 >= 0 || t.osel1 >= 0);
     };
     std::vector<std::vector<int>> runs;  // op indices (pass-relative) of each merged run
     std::vector<GroupPlan> gp(p.group_count);
@@ -129,7 +134,9 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         for (int o = o0; o < o1; ++o) {
             if (P.fold[o - o0]) continue;
             const TileOp& t = ops[p.op_first + o];
-            if (local_diag(t)) {
+            if (outside_diag(t)) {
+                P.uphase.push_back(o);  // commutes with every op of the group
+            } else if (local_diag(t)) {
                 pend.push_back(o);
                 pend_dep |= touch(t);
             } else if (t.kind == kTDiag) {
@@ -142,6 +149,12 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             }
         }
         flush();
+        if (P.uphase.size() >= 2) {
+            P.items.emplace_back(2, 0);
+        } else {
+            for (int o : P.uphase) P.items.emplace_back(0, o);
+            P.uphase.clear();
+        }
     }
 
     std::ostringstream s;
@@ -201,8 +214,22 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         };
         s << "    j_group<K, HB, NT, " << G.spos[0] << ", " << G.spos[1] << ", " << G.spos[2] << ", " << G.spos[3]
           << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
+        auto gbit = [](int b) { return "(int)((gb >> " + std::to_string(b) + ") & 1ull)"; };
         for (const auto& [what, idx] : P.items) {
             s << "      ";
+            if (what == 2) {
+                s << "{\n        double pr = 1.0, pi = 0.0;\n";
+                for (int o : P.uphase) {
+                    const TileOp& t = ops[p.op_first + o];
+                    std::ostringstream on, k;
+                    on << "(gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull";
+                    k << (t.osel0 >= 0 ? gbit(t.osel0) : std::string("0")) << " | ("
+                      << (t.osel1 >= 0 ? gbit(t.osel1) : std::string("0")) << " << 1)";
+                    s << "        j_uphase_mul(pr, pi, " << on.str() << ", " << k.str() << ", op[" << o << "]);\n";
+                }
+                s << "        j_uphase_apply<T>(xr, xi, pr, pi);\n      }\n";
+                continue;
+            }
             if (what == 1) {
                 int dep = 0;
                 for (int o : runs[idx]) dep |= touch(ops[p.op_first + o]);

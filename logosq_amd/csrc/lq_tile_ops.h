@@ -297,6 +297,28 @@ __device__ __forceinline__ void j_phase_mul(double& pr, double& pi, int ts, cons
     pr = nr;
 }
 
+// Diagonal ops that depend only on bits outside the subcube commute with
+// every op of the group: (pr, pi) *= phase m[k] when the op applies to the
+// thread (its outside controls set); the product is applied once.
+__device__ __forceinline__ void j_uphase_mul(double& pr, double& pi, bool on, int k, const TileOp& op) {
+    const double dr = on ? op.m[2 * k] : 1.0, di = on ? op.m[2 * k + 1] : 0.0;
+    const double nr = pr * dr - pi * di;
+    pi = pr * di + pi * dr;
+    pr = nr;
+}
+
+template <typename T>
+__device__ __forceinline__ void j_uphase_apply(T* xr, T* xi, double pr, double pi) {
+    const T r = (T)pr, i = (T)pi;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const T yr = r * xr[c] - i * xi[c];
+        const T yi = r * xi[c] + i * xr[c];
+        xr[c] = yr;
+        xi[c] = yi;
+    }
+}
+
 template <int DEP, typename T, typename V>
 __device__ __forceinline__ void j_diag_tab(T* xr, T* xi, const V* tab) {
 #pragma unroll
