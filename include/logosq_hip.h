@@ -17,8 +17,11 @@
  *
  * Errors: every function returns an lq_status and never aborts; the text of
  * the last error on the calling thread is available from lq_last_error().
- * Threading: calls on distinct state handles are thread safe; calls on one
- * handle must be serialised by the caller (the reference takes &mut State).
+ * Threading: all device work runs on one library stream with shared scratch
+ * buffers, plan buffers and a shared cache of run-time compiled kernels, so
+ * calls must not overlap: issue them from one thread, or serialise them
+ * (the reference's State is used through &mut as well).  Error text is kept
+ * per thread.
  */
 #ifndef LOGOSQ_HIP_H
 #define LOGOSQ_HIP_H

@@ -47,6 +47,7 @@ struct JitKernel {
 
 static int g_jit_mode = LQ_JIT_REPEAT;
 static constexpr uint64_t kJitMinTiles = 1024;
+static constexpr size_t kJitMaxStructures = 4096;
 static std::unordered_map<std::string, std::unique_ptr<JitKernel>> g_jit;
 static long long g_jit_compiled = 0, g_jit_failed = 0, g_jit_launches = 0;
 static double g_jit_seconds = 0.0;
@@ -366,7 +367,10 @@ static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bo
         if (s.normalize || !jit_eligible(s.pass, ninst, any_size || g_jit_mode == LQ_JIT_ALWAYS)) continue;
         std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype);
         auto it = g_jit.find(src);
-        if (it == g_jit.end()) it = g_jit.emplace(std::move(src), std::make_unique<JitKernel>()).first;
+        if (it == g_jit.end()) {
+            if (g_jit.size() >= kJitMaxStructures) continue;  // bounded cache: new structures stay generic
+            it = g_jit.emplace(std::move(src), std::make_unique<JitKernel>()).first;
+        }
         JitKernel* k = it->second.get();
         ++k->seen;
         if (k->failed) continue;
