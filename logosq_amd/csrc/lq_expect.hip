@@ -199,15 +199,16 @@ __global__ void __launch_bounds__(kExpThreads, ((sizeof(V) << K) <= 32768 ? 4 : 
         for (int j = 0; j < geo.hb; ++j) base = insert_zero(base, geo.hpos[j]);
         {
             V r[EPT];
+            constexpr bool FULLT = TS >= (uint32_t)kExpThreads;
 #pragma unroll
             for (int j = 0; j < EPT; ++j) {
                 const uint32_t e = tid + (uint32_t)j * kExpThreads;
-                if (e < TS) r[j] = s[base | (e & lowmask) | hoff[e >> geo.L]];
+                if (FULLT || e < TS) r[j] = s[base | (e & lowmask) | hoff[e >> geo.L]];
             }
 #pragma unroll
             for (int j = 0; j < EPT; ++j) {
                 const uint32_t e = tid + (uint32_t)j * kExpThreads;
-                if (e < TS) tile[swz(e)] = r[j];
+                if (FULLT || e < TS) tile[swz(e)] = r[j];
             }
         }
         __syncthreads();
@@ -234,10 +235,14 @@ __global__ void __launch_bounds__(kExpThreads, ((sizeof(V) << K) <= 32768 ? 4 : 
                 }
                 auto pairs = [&](auto nqc) {
                     constexpr int NQ = decltype(nqc)::value;
+                    // every thread has PPT pairs when the tile holds at least
+                    // one per thread: no bound test, so the unrolled steps'
+                    // LDS reads are all in flight together
+                    constexpr bool FULL = TS / 2 >= (uint32_t)kExpThreads;
 #pragma unroll
                     for (int u = 0; u < PPT; ++u) {
                         const uint32_t k = tid + (uint32_t)u * kExpThreads;
-                        if (k >= TS / 2) break;
+                        if (!FULL && k >= TS / 2) break;
                         const uint32_t e = ins0(k, G.hl);
                         const V a = tile[swz(e)], b = tile[swz(e ^ G.xl)];
                         const double zr = (double)a.x * (double)b.x + (double)a.y * (double)b.y;
