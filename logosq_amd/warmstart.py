@@ -438,6 +438,7 @@ def cmd_time_gradient(a):
     from ._lib import call
     if a.tile_bits:
         call("lq_set_fused_config", a.tile_bits, a.tile_bits + 1, a.low_bits)
+    call("lq_set_expect_tiles", {"off": 0, "auto": 1, "on": 2}[a.expect_tiles])
     ans = ry_rz_cz_ring_ansatz(a.qubits, a.layers)
     obs = heisenberg_ring(a.qubits, 0.3, -0.7, 0.5)
     theta = plain_init(ans.num_parameters(), 5)
@@ -482,6 +483,8 @@ def main(argv=None):
     t.add_argument("--calls", type=int, default=3)
     t.add_argument("--tile-bits", type=int, default=0, help="fused tile of 2^k complex128 amplitudes (default: 12)")
     t.add_argument("--low-bits", type=int, default=5)
+    t.add_argument("--expect-tiles", choices=["off", "auto", "on"], default="auto",
+                   help="flip groups of the expectation from LDS tiles (lq_set_expect_tiles)")
     a = ap.parse_args(argv)
     {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge,
      "time-gradient": cmd_time_gradient}[a.cmd](a)
