@@ -23,3 +23,15 @@ def hip():
     assert rc == 0, lib.lq_last_error()
     print("device:", name.value.decode(), "CUs:", cu.value, flush=True)
     return lib
+
+
+@pytest.fixture(scope="session", autouse=True)
+def forced_paths():
+    """LQ_TEST_FORCE_PATHS=1: run every test with each fused pass on its
+    run-time specialised kernel (compiled at first sight, any size) and every
+    expectation's flip groups from LDS tiles (from 8 qubits), so the whole
+    suite checks those paths against the oracle too."""
+    if os.environ.get("LQ_TEST_FORCE_PATHS") == "1":
+        from tests import modes
+        modes.restore_defaults()
+    yield

@@ -10,6 +10,7 @@ import pytest
 
 import logosq_amd as lq
 from logosq_amd import abi, algorithms
+from tests import modes
 
 
 def plan(n, codes, coeffs, dtype=abi.LQ_C128):
@@ -25,8 +26,9 @@ def plan(n, codes, coeffs, dtype=abi.LQ_C128):
 @pytest.fixture
 def tiles_mode():
     lq.load()
-    yield lambda m: lq._lib.call("lq_set_expect_tiles", m)
     lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
+    yield lambda m: lq._lib.call("lq_set_expect_tiles", m)
+    modes.restore_defaults()
 
 
 def test_ring_30_auto_uses_tiles(tiles_mode):

@@ -14,6 +14,7 @@ import logosq_amd as lq
 from logosq_amd import abi, algorithms
 from logosq_amd import jit
 from tests import bruteforce as bf
+from tests import modes
 from tests.test_gpu_fused import run
 from tests.test_planner import random_circuit
 
@@ -27,7 +28,7 @@ def say(*a):
 @pytest.fixture
 def jit_mode():
     yield jit.set_jit
-    jit.set_jit(jit.REPEAT)
+    modes.restore_defaults()
 
 
 def _launched(before):

@@ -12,6 +12,7 @@ import oracle
 import logosq_amd as lq
 from logosq_amd import abi
 from tests import bruteforce as bf
+from tests import modes
 from tests.test_oracle import all_gates, G, random_codes, ring_template, hea_template
 
 pytestmark = pytest.mark.gpu
@@ -342,12 +343,12 @@ def test_flip_groups_from_tiles(hip, n):
     s64 = lq.State(n, lq.C64)
     s64.set_vector(psi)
     try:
-        for mode in (abi.EXPECT_TILES_ON, abi.EXPECT_TILES_OFF, abi.EXPECT_TILES_AUTO):
+        for mode in (abi.EXPECT_TILES_ON, abi.EXPECT_TILES_OFF):
             lq._lib.call("lq_set_expect_tiles", mode)
             assert abs(obs.expectation(st) - want) <= 1e-10, mode
             assert abs(obs.expectation(s64) - want) <= 1e-5, mode
     finally:
-        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
+        modes.restore_defaults()
 
 
 def test_batched_gradient_flip_groups_from_tiles(hip):
@@ -368,4 +369,4 @@ def test_batched_gradient_flip_groups_from_tiles(hip):
             assert abs(e1 - e0) <= 1e-12
             assert np.max(np.abs(np.array(g1) - np.array(g0))) <= 1e-12
     finally:
-        lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
+        modes.restore_defaults()

@@ -5,13 +5,14 @@ import pytest
 
 import logosq_amd as lq
 from logosq_amd import abi, algorithms, jit
+from tests import modes
 from tests.test_planner import random_circuit
 
 
 def test_jit_mode_rejects_unknown():
     with pytest.raises(lq.LogosQError):
         jit.set_jit(7)
-    jit.set_jit(jit.REPEAT)
+    modes.restore_defaults()
 
 
 @pytest.mark.parametrize("dtype", [abi.LQ_C128, abi.LQ_C64])
