@@ -215,6 +215,7 @@ __global__ void __launch_bounds__(kExpThreads, ((sizeof(V) << K) <= 32768 ? 4 : 
         constexpr int PPT = TS / 2 >= (uint32_t)kExpThreads ? (int)(TS / 2 / kExpThreads) : 1;
         for (int g = 0; g < ngroups; ++g) {
             const ExpTileGroup G = groups[g];
+            const uint32_t sxl = swz(G.xl);
             // terms four at a time, their tile-uniform sign folded into the
             // weights (gr for the real part, gi for the imaginary part)
             for (int q0 = G.first; q0 < G.first + G.count; q0 += 4) {
@@ -244,7 +245,8 @@ __global__ void __launch_bounds__(kExpThreads, ((sizeof(V) << K) <= 32768 ? 4 : 
                         const uint32_t k = tid + (uint32_t)u * kExpThreads;
                         if (!FULL && k >= TS / 2) break;
                         const uint32_t e = ins0(k, G.hl);
-                        const V a = tile[swz(e)], b = tile[swz(e ^ G.xl)];
+                        const uint32_t se = swz(e);
+                        const V a = tile[se], b = tile[se ^ sxl];  // swz is linear over XOR
                         const double zr = (double)a.x * (double)b.x + (double)a.y * (double)b.y;
                         const double zi = (double)a.x * (double)b.y - (double)a.y * (double)b.x;
                         double wr = 0.0, wi = 0.0;
