@@ -176,15 +176,19 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "  const uint64_t total = ninst * p.ntiles;\n";
     s << "  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;\n";
     s << "  const uint64_t w0 = (uint64_t)blockIdx.x * per, w1 = w0 + per < total ? w0 + per : total;\n";
+    if (!runs.empty()) s << "  uint32_t tab_inst = 0xffffffffu;\n";
     s << "  for (uint64_t w = w0; w < w1; ++w) {\n";
     s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));\n";
     s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
     s << "    V* st = psi + (uint64_t)inst * inst_stride;\n";
     s << "    const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;\n";
     if (!runs.empty()) {
-        // phase tables: run r by the sixteen threads of slice r mod NT/16;
-        // the tile load's barrier publishes them
-        s << "    {\n      const int ts = (int)(threadIdx.x & 15u), tg = (int)(threadIdx.x >> 4);\n";
+        // phase tables: run r by the sixteen threads of slice r mod NT/16,
+        // when the block reaches a new instance (its tiles are contiguous,
+        // so mostly once); the tile load's barrier publishes them, and the
+        // previous tile's last barrier has retired every read of the old ones
+        s << "    if (inst != tab_inst) {\n      tab_inst = inst;\n"
+             "      const int ts = (int)(threadIdx.x & 15u), tg = (int)(threadIdx.x >> 4);\n";
         for (size_t r = 0; r < runs.size(); ++r) {
             s << "      if (tg == " << (int)(r % (size_t)(NT / 16)) << ") {\n        double pr = 1.0, pi = 0.0;\n";
             for (int o : runs[r]) {
