@@ -171,6 +171,16 @@ __global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict_
     if (threadIdx.x == 0) partial[b] = t;
 }
 
+// States first .. first+count-1 of the batch become copies of state 0.
+template <typename V>
+__global__ void __launch_bounds__(256) k_copy_base(V* __restrict__ psi, uint64_t dim, uint64_t first, uint64_t count) {
+    const uint64_t total = dim * count;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (uint64_t)gridDim.x * 256) {
+        const uint64_t c = t / dim, i = t - c * dim;
+        psi[(first + c) * dim + i] = psi[i];
+    }
+}
+
 // energies[inst] = sum of the partials of the instance's blocks, then of
 // its LDS-tile partials (tpart[layout][inst][gx]), in a fixed order.
 __global__ void __launch_bounds__(256) k_sum_ranges(const double* __restrict__ partial,
@@ -227,6 +237,10 @@ struct BatchJob {
     std::vector<size_t> lay_off;
     int tile_gx = 0;
     size_t tpart_off = 0;
+    // shared prefixes: instances are ordered by the step where they leave
+    // instance 0's circuit; alive[k] instances take part in step k (the rest
+    // still equal instance 0) and alive[steps] = ninst.  Empty: all, always.
+    std::vector<int> alive;
 };
 
 // Pinned space for the op tables of ninst instances (filled by the caller),
@@ -329,8 +343,25 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
 
 // The device work of a prepared job (capturable when the plan has no
 // renormalisation step).
+static int copy_base(const BatchJob& j, void* psi, int first, int count) {
+    if (count <= 0) return LQ_OK;
+    const uint64_t total = j.dim * (uint64_t)count;
+    const unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+    if (j.dtype == LQ_C128)
+        hipLaunchKernelGGL(k_copy_base<double2>, dim3(g), dim3(256), 0, stream(), (double2*)psi, j.dim,
+                           (uint64_t)first, (uint64_t)count);
+    else
+        hipLaunchKernelGGL(k_copy_base<float2>, dim3(g), dim3(256), 0, stream(), (float2*)psi, j.dim,
+                           (uint64_t)first, (uint64_t)count);
+    LQ_LAUNCH_CHECK();
+    return LQ_OK;
+}
+
 static int job_launch(const BatchJob& j, void* psi) {
-    const uint64_t total = j.dim * (uint64_t)j.ninst;
+    const size_t nsteps = j.plan->steps.size();
+    auto alive = [&](size_t k) { return j.alive.empty() ? j.ninst : j.alive[k]; };
+    // instances that diverge in the first step start from |0...0> too
+    const uint64_t total = j.dim * (uint64_t)alive(0);
     const unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
     if (j.dtype == LQ_C128)
         hipLaunchKernelGGL(k_fill_zero_states<double2>, dim3(g), dim3(256), 0, stream(), (double2*)psi, j.dim, total);
@@ -340,11 +371,18 @@ static int job_launch(const BatchJob& j, void* psi) {
     LQ_HIP_CHECK(hipMemcpyAsync(g_bplan.p, g_hplan.p, j.plan_bytes, hipMemcpyHostToDevice, stream()));
     const TileOp* d_ops = (const TileOp*)g_bplan.p;
     const TileGroup* d_groups = (const TileGroup*)((char*)g_bplan.p + j.goff);
-    for (const Step& s : j.plan->steps) {
+    for (size_t k = 0; k < nsteps; ++k) {
+        const Step& s = j.plan->steps[k];
+        const int na = alive(k);
         int rc;
+        if (k > 0 && na > alive(k - 1)) {
+            // instances leaving instance 0's circuit at this step: its state so far
+            rc = copy_base(j, psi, alive(k - 1), na - alive(k - 1));
+            if (rc) return rc;
+        }
         if (s.normalize) {
             // MatrixGate renormalisation: per instance
-            for (int i = 0; i < j.ninst; ++i) {
+            for (int i = 0; i < na; ++i) {
                 lq_state tmp;
                 tmp.n = j.n;
                 tmp.dtype = j.dtype;
@@ -356,7 +394,11 @@ static int job_launch(const BatchJob& j, void* psi) {
             }
             continue;
         }
-        rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)j.ninst, j.dim, j.nops, s.jit);
+        rc = launch_tile_pass(psi, j.dtype, s.pass, d_ops, d_groups, (uint64_t)na, j.dim, j.nops, s.jit);
+        if (rc) return rc;
+    }
+    if (nsteps > 0 && alive(nsteps - 1) < j.ninst) {
+        int rc = copy_base(j, psi, alive(nsteps - 1), j.ninst - alive(nsteps - 1));
         if (rc) return rc;
     }
     if (j.blocks > 0 || !j.lays.empty()) {
@@ -427,6 +469,8 @@ static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
         k.push_back(p.ntiles);
         k.push_back((uint64_t)(uintptr_t)s.jit);
     }
+    k.push_back(j.alive.size());
+    for (int a : j.alive) k.push_back((uint64_t)a);
     for (const ExpLayout& l : j.lays) {
         k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
         for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
@@ -561,6 +605,33 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
     std::vector<PauliSum> sums(1, ps);
     std::vector<double> p(params, params + num_params);
     std::vector<Prim> one;
+    // Shared prefixes (one chunk only): the circuit shifted in parameter i
+    // equals instance 0's up to the step holding the first op of a gate
+    // that uses it.  Instances run in the order of that step, each joins as
+    // a copy of instance 0's state there, and every step runs over the
+    // instances that have left instance 0's circuit by then.
+    const size_t nsteps = plan.steps.size();
+    std::vector<int> perm(ninst);
+    for (int i = 0; i < ninst; ++i) perm[i] = i;
+    std::vector<int> alive;
+    if (chunk >= ninst && nsteps > 0) {
+        std::vector<int> step_of_op(nops, (int)nsteps);
+        for (size_t k = 0; k < nsteps; ++k)
+            if (!plan.steps[k].normalize)
+                for (int o = 0; o < plan.steps[k].pass.op_count; ++o)
+                    step_of_op[plan.steps[k].pass.op_first + o] = (int)k;
+        std::vector<int> dstep(ninst, -1);  // instance 0 leads
+        for (int i = 0; i < num_params; ++i) {
+            int d = (int)nsteps;
+            for (int g : users[i])
+                if (op_of[g] >= 0) d = std::min(d, step_of_op[op_of[g]]);
+            dstep[2 * i + 1] = dstep[2 * i + 2] = d;
+        }
+        std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return dstep[a] < dstep[b]; });
+        alive.assign(nsteps + 1, 1);
+        for (int i = 1; i < ninst; ++i)
+            for (size_t k = (size_t)dstep[i]; k <= nsteps; ++k) alive[k]++;
+    }
     for (int c0 = 0; c0 < ninst; c0 += chunk) {
         const int cn = std::min(chunk, ninst - c0);
         BatchJob job;
@@ -570,11 +641,12 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
         job.dim = dim;
         job.plan = &plan;
         job.nops = nops;
+        if (cn == ninst) job.alive = alive;
         TileOp* ops = nullptr;
         rc = job_ops(job, &ops);
         if (rc) return rc;
         for (int k = 0; k < cn; ++k) {
-            const int inst = c0 + k;
+            const int inst = perm[c0 + k];
             TileOp* o = ops + nops * (size_t)k;
             std::memcpy(o, plan.ops.data(), nops * sizeof(TileOp));
             if (inst == 0) continue;
@@ -594,10 +666,12 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
             p[i] = params[i];
         }
         std::vector<int> obs_of(cn, 0);
+        std::vector<double> e(cn);
         rc = job_expect(job, sums, obs_of);
         if (!rc) rc = g_states.get((size_t)cn * dim * amp);
-        if (!rc) rc = job_run(job, g_states.p, energies.data() + c0);
+        if (!rc) rc = job_run(job, g_states.p, e.data());
         if (rc) return rc;
+        for (int k = 0; k < cn; ++k) energies[perm[c0 + k]] = e[k];
     }
     (void)fp;
     if (energy_out) *energy_out = energies[0];
