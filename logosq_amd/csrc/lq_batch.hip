@@ -120,7 +120,49 @@ __global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict_
     const PauliTermDev* tm = terms + it.term_first;
     const uint64_t stride = (uint64_t)it.nblocks * kEThreads;
     double acc = 0.0;
-    if (it.x == 0) {
+    if (it.x == 0 && (stride & (stride - 1)) == 0) {
+        // A thread visits i = i0 | u * stride (stride a power of two above
+        // i0), so a term's sign (-1)^{popc(i & zy)} splits into i0's part,
+        // folded into the coefficient once per term, and the step's part,
+        // the same for the whole wavefront (scalar unit).  Eight steps'
+        // probabilities stay in registers across the term chunks.
+        const uint64_t i0 = (uint64_t)lb * kEThreads + threadIdx.x;
+        for (uint64_t u0 = 0; i0 + u0 * stride < it.work; u0 += 8) {
+            double pv[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const uint64_t i = i0 + (u0 + v) * stride;
+                pv[v] = 0.0;
+                if (i < it.work) {
+                    const V a = s[i];
+                    pv[v] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+                }
+            }
+            for (int q0 = 0; q0 < it.term_count; q0 += 8) {
+                const int nq = it.term_count - q0 < 8 ? it.term_count - q0 : 8;
+                double cp[8];
+                uint64_t zy[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    cp[j] = 0.0;
+                    zy[j] = 0;
+                    if (j < nq) {
+                        const PauliTermDev t = tm[q0 + j];
+                        zy[j] = t.zy;
+                        cp[j] = (__popcll(i0 & t.zy) & 1) ? -t.cr : t.cr;
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const uint64_t su = (u0 + v) * stride;
+                    double w = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) w += (__popcll(su & zy[j]) & 1) ? -cp[j] : cp[j];
+                    acc += pv[v] * w;
+                }
+            }
+        }
+    } else if (it.x == 0) {
         for (uint64_t i = (uint64_t)lb * kEThreads + threadIdx.x; i < it.work; i += stride) {
             V a = s[i];
             double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
