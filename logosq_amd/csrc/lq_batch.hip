@@ -87,6 +87,13 @@ struct ExpItem {
     int32_t inst;
 };
 
+// One op whose matrix differs from instance 0's in instance `inst`.
+struct OpOverride {
+    int32_t inst, op;
+    double m[32];
+};
+static_assert(sizeof(TileOp) % 8 == 0, "op tables are replicated in 8-byte words");
+
 constexpr int kEThreads = 256;
 
 template <int NT>
@@ -213,6 +220,21 @@ __global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict_
     if (threadIdx.x == 0) partial[b] = t;
 }
 
+// Every instance's op table becomes a copy of instance 0's (8-byte words).
+__global__ void __launch_bounds__(256) k_replicate_ops(uint64_t* __restrict__ dst, const uint64_t* __restrict__ base,
+                                                      uint64_t words, uint64_t ninst) {
+    const uint64_t total = words * ninst;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (uint64_t)gridDim.x * 256)
+        dst[t] = base[t % words];
+}
+
+// One block per override: instance inst's op gets its own matrix.
+__global__ void __launch_bounds__(64) k_apply_overrides(TileOp* __restrict__ ops, const OpOverride* __restrict__ ov,
+                                                       uint64_t nops) {
+    const OpOverride& o = ov[blockIdx.x];
+    if (threadIdx.x < 32) ops[(uint64_t)o.inst * nops + o.op].m[threadIdx.x] = o.m[threadIdx.x];
+}
+
 // States first .. first+count-1 of the batch become copies of state 0.
 template <typename V>
 __global__ void __launch_bounds__(256) k_copy_base(V* __restrict__ psi, uint64_t dim, uint64_t first, uint64_t count) {
@@ -283,7 +305,37 @@ struct BatchJob {
     // instance 0's circuit; alive[k] instances take part in step k (the rest
     // still equal instance 0) and alive[steps] = ninst.  Empty: all, always.
     std::vector<int> alive;
+    // compact op tables: the host holds instance 0's table and per-instance
+    // overrides (hbase bytes, then nover records); the device replicates the
+    // table into every instance's slot and applies the overrides
+    bool replicate = false;
+    size_t nover = 0, hgoff = 0, hover = 0, hbytes = 0, stage = 0;
 };
+
+
+// Compact op tables: pinned [instance 0's ops][groups][nover overrides]
+// (filled by the caller: *base and *over); the device holds every
+// instance's table, the groups, then a staging copy of the host layout.
+static int job_ops_compact(BatchJob& j, size_t nover, TileOp** base, OpOverride** over) {
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t ob = j.nops * (size_t)j.ninst * sizeof(TileOp);
+    j.replicate = true;
+    j.nover = nover;
+    j.goff = al(ob);
+    const size_t gb = j.plan->groups.size() * sizeof(TileGroup);
+    j.plan_bytes = j.goff + gb;
+    j.hgoff = al(j.nops * sizeof(TileOp));
+    j.hover = al(j.hgoff + gb);
+    j.hbytes = j.hover + nover * sizeof(OpOverride);
+    j.stage = al(j.plan_bytes);
+    int rc = g_hplan.get(j.hbytes + 256);
+    if (!rc) rc = g_bplan.get(j.stage + j.hbytes + 256);
+    if (rc) return rc;
+    if (gb) std::memcpy((char*)g_hplan.p + j.hgoff, j.plan->groups.data(), gb);
+    *base = (TileOp*)g_hplan.p;
+    *over = (OpOverride*)((char*)g_hplan.p + j.hover);
+    return LQ_OK;
+}
 
 // Pinned space for the op tables of ninst instances (filled by the caller),
 // followed by the plan's groups.
@@ -410,7 +462,26 @@ static int job_launch(const BatchJob& j, void* psi) {
     else
         hipLaunchKernelGGL(k_fill_zero_states<float2>, dim3(g), dim3(256), 0, stream(), (float2*)psi, j.dim, total);
     LQ_LAUNCH_CHECK();
-    LQ_HIP_CHECK(hipMemcpyAsync(g_bplan.p, g_hplan.p, j.plan_bytes, hipMemcpyHostToDevice, stream()));
+    if (j.replicate) {
+        char* dev = (char*)g_bplan.p;
+        LQ_HIP_CHECK(hipMemcpyAsync(dev + j.stage, g_hplan.p, j.hbytes, hipMemcpyHostToDevice, stream()));
+        const size_t gb = j.plan->groups.size() * sizeof(TileGroup);
+        if (gb)
+            LQ_HIP_CHECK(hipMemcpyAsync(dev + j.goff, dev + j.stage + j.hgoff, gb, hipMemcpyDeviceToDevice, stream()));
+        const uint64_t words = j.nops * sizeof(TileOp) / 8;
+        const uint64_t tw = words * (uint64_t)j.ninst;
+        const unsigned g2 = (unsigned)std::min<uint64_t>((tw + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_replicate_ops, dim3(g2), dim3(256), 0, stream(), (uint64_t*)dev,
+                           (const uint64_t*)(dev + j.stage), words, (uint64_t)j.ninst);
+        LQ_LAUNCH_CHECK();
+        if (j.nover) {
+            hipLaunchKernelGGL(k_apply_overrides, dim3((unsigned)j.nover), dim3(64), 0, stream(), (TileOp*)dev,
+                               (const OpOverride*)(dev + j.stage + j.hover), (uint64_t)j.nops);
+            LQ_LAUNCH_CHECK();
+        }
+    } else {
+        LQ_HIP_CHECK(hipMemcpyAsync(g_bplan.p, g_hplan.p, j.plan_bytes, hipMemcpyHostToDevice, stream()));
+    }
     const TileOp* d_ops = (const TileOp*)g_bplan.p;
     const TileGroup* d_groups = (const TileGroup*)((char*)g_bplan.p + j.goff);
     for (size_t k = 0; k < nsteps; ++k) {
@@ -513,6 +584,10 @@ static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
     }
     k.push_back(j.alive.size());
     for (int a : j.alive) k.push_back((uint64_t)a);
+    k.push_back(j.replicate ? 1 : 0);
+    k.push_back(j.nover);
+    k.push_back(j.hbytes);
+    k.push_back(j.stage);
     for (const ExpLayout& l : j.lays) {
         k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
         for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
@@ -684,6 +759,48 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
         job.plan = &plan;
         job.nops = nops;
         if (cn == ninst) job.alive = alive;
+        auto shifted_prims = [&](int inst, int g, std::vector<Prim>& out) {
+            // instance 2i+1: theta_i + s, instance 2i+2: theta_i - s
+            const int i = (inst - 1) / 2;
+            const double sh = (inst % 2 == 1) ? shift : -shift;
+            p[i] = params[i] + sh;
+            lq_gate gg = tmpl[g];
+            for (int kk = 0; kk < 3; ++kk)
+                if (gg.param[kk] >= 0) gg.theta[kk] = p[gg.param[kk]];
+            p[i] = params[i];
+            out.clear();
+            return gates_to_prims(n, &gg, 1, out);
+        };
+        if (cn == ninst) {
+            // one batch: instance 0's table and each instance's shifted ops
+            size_t nover = 0;
+            for (int k = 1; k < cn; ++k)
+                for (int g : users[(perm[k] - 1) / 2]) nover += op_of[g] >= 0;
+            TileOp* base_ops = nullptr;
+            OpOverride* over = nullptr;
+            rc = job_ops_compact(job, nover, &base_ops, &over);
+            if (rc) return rc;
+            std::memcpy(base_ops, plan.ops.data(), nops * sizeof(TileOp));
+            size_t q = 0;
+            for (int k = 1; k < cn; ++k)
+                for (int g : users[(perm[k] - 1) / 2]) {
+                    if (op_of[g] < 0) continue;
+                    rc = shifted_prims(perm[k], g, one);
+                    if (rc) return rc;
+                    over[q].inst = k;
+                    over[q].op = op_of[g];
+                    std::memcpy(over[q].m, one[0].m, sizeof(over[q].m));
+                    ++q;
+                }
+            std::vector<int> obs_of(cn, 0);
+            std::vector<double> e(cn);
+            rc = job_expect(job, sums, obs_of);
+            if (!rc) rc = g_states.get((size_t)cn * dim * amp);
+            if (!rc) rc = job_run(job, g_states.p, e.data());
+            if (rc) return rc;
+            for (int k = 0; k < cn; ++k) energies[perm[c0 + k]] = e[k];
+            continue;
+        }
         TileOp* ops = nullptr;
         rc = job_ops(job, &ops);
         if (rc) return rc;
