@@ -206,3 +206,49 @@ def test_vqe_adam_batch_equals_single_runs(hip):
     xo, eo, ho, co, _ = oracle.vqe_run(n, ans.template(), ans.num_parameters(), codes, coeffs, X0[0], 1, 0.05, 25)
     assert np.max(np.abs(np.array(res[0].convergence_history) - ho)) <= 1e-9
     assert abs(res[0].ground_state_energy - eo) <= 1e-9
+
+
+def test_batched_param_shift_shared_prefix_edges(hip):
+    """Shared prefixes at their edges: parameter 0 drives the first and the
+    last gate, parameter 1 no gate (its shifted circuits equal the base one
+    and join only after the last step), parameter 2 only the last gate, the
+    rest a brickwork in between.  Batched against sequential and the
+    oracle."""
+    print("batched parameter shift, shared-prefix edges", flush=True)
+    n = 16  # wider than one tile: the plan has several steps
+
+    def build(p):
+        c = lq.Circuit(n)
+        c.ry(0, p[0])
+        for layer in range(3):
+            for q in range(n):
+                c.rx(q, p[3 + layer * n + q])
+            for q in range(layer % 2, n - 1, 2):
+                c.cnot(q, q + 1)
+        c.rz(5, p[2])
+        c.ry(n - 1, p[0])
+        return c
+
+    ans = lq.ParameterizedCircuit(n, 3 + 3 * n, build)
+    assert algorithms_passes(n, ans.template()) >= 2
+    obs = lq.heisenberg_ring(n, 0.5, -0.6, 0.7)
+    x = np.random.default_rng(92).uniform(0, 2 * math.pi, ans.num_parameters())
+    e_b, g_b = lq.ParameterShift(batched=True).compute_gradient(ans, obs, x, energy=True)
+    e_s, g_s = lq.ParameterShift(batched=False).compute_gradient(ans, obs, x, energy=True)
+    assert abs(e_b - e_s) <= 1e-12
+    assert np.max(np.abs(np.array(g_b) - np.array(g_s))) <= 1e-12
+    assert abs(g_b[1]) <= 1e-12  # the unused parameter
+    codes, coeffs = obs.codes()
+    want = oracle.param_shift(n, ans.template(), x, codes, coeffs)
+    assert np.max(np.abs(np.array(g_b) - want)) <= 1e-10
+
+
+def algorithms_passes(n, gates):
+    import ctypes
+    lib = lq.load()
+    order = (ctypes.c_int * len(gates))()
+    npass = ctypes.c_int()
+    assert lib.lq_plan_fused(n, abi.LQ_C128, abi.gate_array(gates), len(gates), 0, 0, order, None,
+                             ctypes.byref(npass)) == 0
+    return npass.value
+
