@@ -439,6 +439,7 @@ def cmd_time_gradient(a):
     if a.tile_bits:
         call("lq_set_fused_config", a.tile_bits, a.tile_bits + 1, a.low_bits)
     call("lq_set_expect_tiles", {"off": 0, "auto": 1, "on": 2}[a.expect_tiles])
+    call("lq_set_jit", {"off": 0, "repeat": 1, "always": 2}[a.jit])
     ans = ry_rz_cz_ring_ansatz(a.qubits, a.layers)
     obs = heisenberg_ring(a.qubits, 0.3, -0.7, 0.5)
     theta = plain_init(ans.num_parameters(), 5)
@@ -485,6 +486,8 @@ def main(argv=None):
     t.add_argument("--low-bits", type=int, default=5)
     t.add_argument("--expect-tiles", choices=["off", "auto", "on"], default="auto",
                    help="flip groups of the expectation from LDS tiles (lq_set_expect_tiles)")
+    t.add_argument("--jit", choices=["off", "repeat", "always"], default="repeat",
+                   help="run-time specialised tile passes (lq_set_jit)")
     a = ap.parse_args(argv)
     {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge,
      "time-gradient": cmd_time_gradient}[a.cmd](a)
