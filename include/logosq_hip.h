@@ -17,10 +17,13 @@
  *
  * Errors: every function returns an lq_status and never aborts; the text of
  * the last error on the calling thread is available from lq_last_error().
- * Threading: all device work runs on one library stream with shared scratch
- * buffers, plan buffers and a shared cache of run-time compiled kernels, so
- * calls must not overlap: issue them from one thread, or serialise them
- * (the reference's State is used through &mut as well).  Error text is kept
+ * Threading: calls on distinct handles may come from different threads.
+ * All device work runs on one library stream with shared scratch buffers,
+ * plan buffers and a shared cache of run-time compiled kernels, so every
+ * entry point takes one library-wide lock and concurrent calls run one at a
+ * time (recursive: an optimiser's objective callback may call back into the
+ * library on the same thread).  One handle must not be used by two threads
+ * at once (the reference's State is used through &mut).  Error text is kept
  * per thread.
  */
 #ifndef LOGOSQ_HIP_H

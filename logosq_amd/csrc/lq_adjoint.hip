@@ -924,6 +924,7 @@ static int adjoint_entry(int num_qubits, int dtype, const lq_gate* tmpl, int num
 extern "C" int lq_adjoint_gradient(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates,
                                    const double* params, int num_params, const uint8_t* paulis, const double* coeffs,
                                    int num_terms, int convention, double* energy_out, double* grad_out) {
+    LQ_API_GUARD;
     std::vector<double> dummy;
     if (num_params == 0) dummy.resize(1);
     return adjoint_entry(num_qubits, dtype, tmpl, num_gates, 1, params, num_params, paulis, coeffs, num_terms,
@@ -934,6 +935,7 @@ extern "C" int lq_adjoint_gradient_batch(int num_qubits, int dtype, const lq_gat
                                          const double* params, int num_params, const uint8_t* paulis,
                                          const double* coeffs, int num_terms, int convention, double* energies,
                                          double* grads) {
+    LQ_API_GUARD;
     return adjoint_entry(num_qubits, dtype, tmpl, num_gates, count, params, num_params, paulis, coeffs, num_terms,
                          convention, energies, grads);
 }

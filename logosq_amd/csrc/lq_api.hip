@@ -146,6 +146,7 @@ using namespace lq;
 extern "C" {
 
 int lq_apply_gate(lq_state* st, const lq_gate* gate) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -157,6 +158,7 @@ int lq_apply_gate(lq_state* st, const lq_gate* gate) {
 }
 
 int lq_apply_1q(lq_state* st, int qubit, const double m[8]) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -170,6 +172,7 @@ int lq_apply_1q(lq_state* st, int qubit, const double m[8]) {
 }
 
 int lq_apply_ctrl_1q(lq_state* st, const int* controls, int num_controls, int target, const double m[8]) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -193,6 +196,7 @@ int lq_apply_ctrl_1q(lq_state* st, const int* controls, int num_controls, int ta
 }
 
 int lq_apply_2q(lq_state* st, int q_a, int q_b, const double m[32]) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -208,6 +212,7 @@ int lq_apply_2q(lq_state* st, int q_a, int q_b, const double m[32]) {
 }
 
 int lq_apply_dense(lq_state* st, const double* matrix, uint64_t dim) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -241,6 +246,7 @@ int lq_apply_dense(lq_state* st, const double* matrix, uint64_t dim) {
 }
 
 int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;

@@ -863,6 +863,7 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
                                      const int* num_gates, const uint8_t* const* paulis,
                                      const double* const* coeffs, const int* num_terms, int convention, int dtype,
                                      double* energies) {
+    LQ_API_GUARD;
     clear_error();
     if (count < 0 || (count > 0 && (!num_qubits || !circuits || !num_gates || !paulis || !coeffs || !num_terms ||
                                     !energies))) {
@@ -946,12 +947,14 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
 }
 
 extern "C" int lq_set_batch_graphs(int enable) {
+    LQ_API_GUARD;
     clear_error();
     g_graphs_on = enable != 0;
     return LQ_OK;
 }
 
 extern "C" int lq_batch_graph_stats(long long* captures, long long* replays) {
+    LQ_API_GUARD;
     clear_error();
     if (captures) *captures = g_graph_captures;
     if (replays) *replays = g_graph_replays;

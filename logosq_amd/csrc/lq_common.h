@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/logosq_hip.h"
@@ -87,6 +88,14 @@ namespace lq {
 // Thread-local error text for lq_last_error().
 void set_error(const char* fmt, ...);
 void clear_error();
+
+// One library-wide lock taken at every C-ABI entry point.  All device work
+// shares one stream, scratch buffers, plan buffers and the run-time compiled
+// kernel cache, so calls from several threads (on distinct handles) run one
+// at a time instead of racing.  Recursive: a host callback invoked by an
+// optimiser entry point may call back into the library on the same thread.
+std::recursive_mutex& api_mutex();
+#define LQ_API_GUARD std::lock_guard<std::recursive_mutex> lq_api_guard_(::lq::api_mutex())
 
 // The library's single HIP stream (default priority) and a small device
 // scratch area for reductions, created on first use.

@@ -629,6 +629,7 @@ using namespace lq;
 
 extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
                               int convention, int* tile_layouts, int* pair_groups, int* diag_groups) {
+    LQ_API_GUARD;
     clear_error();
     if (num_qubits < 1 || num_qubits > kMaxQubits || num_terms < 0 || (num_terms > 0 && (!paulis || !coeffs)) ||
         (dtype != LQ_C128 && dtype != LQ_C64)) {
@@ -650,6 +651,7 @@ extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, 
 }
 
 extern "C" int lq_set_expect_tiles(int mode) {
+    LQ_API_GUARD;
     clear_error();
     if (mode != LQ_EXPECT_TILES_OFF && mode != LQ_EXPECT_TILES_AUTO && mode != LQ_EXPECT_TILES_ON) {
         set_error("mode must be LQ_EXPECT_TILES_OFF, LQ_EXPECT_TILES_AUTO or LQ_EXPECT_TILES_ON");
@@ -661,6 +663,7 @@ extern "C" int lq_set_expect_tiles(int mode) {
 
 extern "C" int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
                                    int convention, double* out) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;

@@ -851,12 +851,14 @@ void get_fused_config(int* tile_c128, int* tile_c64, int* low) {
 }  // namespace lq
 
 extern "C" int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits) {
+    LQ_API_GUARD;
     clear_error();
     return set_fused_config_internal(tile_bits_c128, tile_bits_c64, low_bits);
 }
 
 extern "C" int lq_plan_fused(int num_qubits, int dtype, const lq_gate* gates, int count, int tile_bits,
                              int low_bits, int* gate_order, int* pass_index, int* num_passes) {
+    LQ_API_GUARD;
     clear_error();
     if (num_qubits < 1 || num_qubits > kMaxQubits || count < 0 || (count > 0 && (!gates || !gate_order)) ||
         !num_passes) {

@@ -58,6 +58,7 @@ extern "C" int lq_param_shift_gradient(int num_qubits, int dtype, const lq_gate*
                                        const double* params, int num_params, double shift, const uint8_t* paulis,
                                        const double* coeffs, int num_terms, int convention, int mode,
                                        double* energy_out, double* grad_out) {
+    LQ_API_GUARD;
     clear_error();
     if (num_qubits < 1 || num_qubits > kMaxQubits || num_params < 0 || (num_params > 0 && (!params || !grad_out))) {
         set_error("invalid arguments");

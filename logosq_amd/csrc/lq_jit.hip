@@ -435,6 +435,7 @@ int launch_jit(const JitKernel* k, void* psi, const TilePass& p, const TileOp* d
 using namespace lq;
 
 extern "C" int lq_set_jit(int mode) {
+    LQ_API_GUARD;
     clear_error();
     if (mode != LQ_JIT_OFF && mode != LQ_JIT_REPEAT && mode != LQ_JIT_ALWAYS) {
         set_error("jit mode must be LQ_JIT_OFF, LQ_JIT_REPEAT or LQ_JIT_ALWAYS");
@@ -445,6 +446,7 @@ extern "C" int lq_set_jit(int mode) {
 }
 
 extern "C" int lq_jit_stats(long long* compiled, long long* failed, long long* launches, double* compile_seconds) {
+    LQ_API_GUARD;
     clear_error();
     if (compiled) *compiled = g_jit_compiled;
     if (failed) *failed = g_jit_failed;
@@ -454,6 +456,7 @@ extern "C" int lq_jit_stats(long long* compiled, long long* failed, long long* l
 }
 
 extern "C" int lq_jit_compile_plan(int num_qubits, int dtype, const lq_gate* gates, int count, int* kernels) {
+    LQ_API_GUARD;
     clear_error();
     if (num_qubits < 1 || num_qubits > kMaxQubits || count < 0 || (count > 0 && !gates) ||
         (dtype != LQ_C128 && dtype != LQ_C64)) {

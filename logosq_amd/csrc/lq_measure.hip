@@ -186,6 +186,7 @@ using namespace lq;
 extern "C" {
 
 int lq_bit_probabilities(const lq_state* st, double* p1) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -197,6 +198,7 @@ int lq_bit_probabilities(const lq_state* st, double* p1) {
 }
 
 int lq_measure_qubit(lq_state* st, int qubit_idx, double u, int* result) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -227,6 +229,7 @@ int lq_measure_qubit(lq_state* st, int qubit_idx, double u, int* result) {
 }
 
 int lq_sample_shots(const lq_state* st, uint64_t seed, int64_t shots, uint64_t* out) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;

@@ -304,6 +304,7 @@ using namespace lq;
 extern "C" {
 
 int lq_mlp_create(int in_dim, int hidden, int out_dim, uint64_t seed, lq_mlp** out) {
+    LQ_API_GUARD;
     clear_error();
     if (!out || in_dim < 1 || in_dim > 8 || hidden < 16 || hidden > 4096 || out_dim < 1 || out_dim > 8192) {
         set_error("mlp sizes out of range (in <= 8, 16 <= hidden <= 4096, out <= 8192)");
@@ -352,6 +353,7 @@ int lq_mlp_create(int in_dim, int hidden, int out_dim, uint64_t seed, lq_mlp** o
 }
 
 int lq_mlp_destroy(lq_mlp* n) {
+    LQ_API_GUARD;
     if (!n) return LQ_OK;
     (void)hipStreamSynchronize(stream());
     float* bufs[] = {n->w, n->g, n->m, n->v, n->x, n->t, n->z1, n->a1, n->z2, n->a2, n->y, n->d3, n->d2, n->d1, n->loss};
@@ -362,12 +364,14 @@ int lq_mlp_destroy(lq_mlp* n) {
 }
 
 int lq_mlp_num_params(const lq_mlp* n, int64_t* count) {
+    LQ_API_GUARD;
     if (!n || !count) return LQ_ERR_INVALID_ARGUMENT;
     *count = n->nparams;
     return LQ_OK;
 }
 
 int lq_mlp_get_params(const lq_mlp* n, float* out, int64_t count) {
+    LQ_API_GUARD;
     clear_error();
     if (!n || !out || count != n->nparams) {
         set_error("parameter buffer must hold exactly %lld floats", (long long)(n ? n->nparams : 0));
@@ -379,6 +383,7 @@ int lq_mlp_get_params(const lq_mlp* n, float* out, int64_t count) {
 }
 
 int lq_mlp_set_params(lq_mlp* n, const float* in, int64_t count) {
+    LQ_API_GUARD;
     clear_error();
     if (!n || !in || count != n->nparams) {
         set_error("parameter buffer must hold exactly %lld floats", (long long)(n ? n->nparams : 0));
@@ -394,6 +399,7 @@ int lq_mlp_set_params(lq_mlp* n, const float* in, int64_t count) {
 }
 
 int lq_mlp_forward(lq_mlp* n, const float* x, int rows, float* y) {
+    LQ_API_GUARD;
     clear_error();
     if (!n || !x || !y || rows < 1) {
         set_error("invalid arguments");
@@ -411,6 +417,7 @@ int lq_mlp_forward(lq_mlp* n, const float* x, int rows, float* y) {
 
 // Loss and parameter gradients of the mean squared error (no update).
 int lq_mlp_gradients(lq_mlp* n, const float* x, const float* target, int rows, float* loss, float* grad) {
+    LQ_API_GUARD;
     clear_error();
     if (!n || !x || !target || rows < 1) {
         set_error("invalid arguments");
@@ -434,6 +441,7 @@ int lq_mlp_gradients(lq_mlp* n, const float* x, const float* target, int rows, f
 // SplitMix64(seed)); loss_history (nullable) gets the mean loss per epoch.
 int lq_mlp_train(lq_mlp* n, const float* x, const float* target, int count, int batch, int epochs, float lr,
                  uint64_t seed, float* loss_history) {
+    LQ_API_GUARD;
     clear_error();
     if (!n || !x || !target || count < 1 || batch < 1 || epochs < 0) {
         set_error("invalid arguments");

@@ -131,6 +131,7 @@ using namespace lq;
 extern "C" {
 
 int lq_noise_amplitude_damping(lq_state* st, const int* qubits, int count, double gamma) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (!rc) rc = check_noise_qubits(st, qubits, count);
@@ -148,6 +149,7 @@ int lq_noise_amplitude_damping(lq_state* st, const int* qubits, int count, doubl
 }
 
 int lq_noise_phase_damping(lq_state* st, const int* qubits, int count, double lambda) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (!rc) rc = check_noise_qubits(st, qubits, count);
@@ -165,6 +167,7 @@ int lq_noise_phase_damping(lq_state* st, const int* qubits, int count, double la
 
 int lq_noise_depolarizing(lq_state* st, const int* qubits, int count, double p, uint64_t* rng_state,
                           int* errors) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (!rc) rc = check_noise_qubits(st, qubits, count);

@@ -183,6 +183,7 @@ extern "C" {
 
 int lq_gd_minimize(double learning_rate, int max_iterations, double tolerance, lq_objective_fn f, lq_gradient_fn g,
                    void* user, double* x, int dim, double* history, lq_opt_result* result) {
+    LQ_API_GUARD;
     clear_error();
     if (!f || !g || !result || dim < 0 || (dim > 0 && !x)) {
         set_error("invalid optimiser arguments");
@@ -195,6 +196,7 @@ int lq_gd_minimize(double learning_rate, int max_iterations, double tolerance, l
 int lq_adam_minimize(double learning_rate, double beta1, double beta2, double epsilon, int max_iterations,
                      double tolerance, lq_objective_fn f, lq_gradient_fn g, void* user, double* x, int dim,
                      double* history, lq_opt_result* result) {
+    LQ_API_GUARD;
     clear_error();
     if (!f || !g || !result || dim < 0 || (dim > 0 && !x)) {
         set_error("invalid optimiser arguments");
@@ -209,6 +211,7 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
                const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
                int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
                double* params, double* history, lq_state* final_state, lq_opt_result* result) {
+    LQ_API_GUARD;
     clear_error();
     if (!result || num_params < 0 || (num_params > 0 && !params)) {
         set_error("invalid VQE arguments");
@@ -260,6 +263,7 @@ int lq_vqe_adam_batch(int num_qubits, int dtype, const lq_gate* tmpl, int num_ga
                       const uint8_t* paulis, const double* coeffs, int num_terms, int convention, int count,
                       double learning_rate, int max_iterations, double tolerance, double* params, double* history,
                       double* energies, int32_t* iterations, int32_t* converged) {
+    LQ_API_GUARD;
     clear_error();
     if (count < 0 || num_params < 0 || max_iterations < 0 || (count > 0 && (!params || !energies || !iterations ||
                                                                             !converged || !coeffs))) {

@@ -22,6 +22,11 @@ void set_error(const char* fmt, ...) {
 
 void clear_error() { g_err[0] = 0; }
 
+std::recursive_mutex& api_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 // ---- library state ----------------------------------------------------------------
 static std::mutex g_mu;
 static bool g_inited = false;
@@ -318,6 +323,7 @@ const char* lq_last_error(void) { return g_err; }
 int lq_abi_version(void) { return LQ_ABI_VERSION; }
 
 int lq_device_info(char* name, size_t name_len, int* cu_count, size_t* lds_bytes, size_t* global_bytes) {
+    LQ_API_GUARD;
     int rc = ensure_init();
     if (rc) return rc;
     int dev = 0;
@@ -334,6 +340,7 @@ int lq_device_info(char* name, size_t name_len, int* cu_count, size_t* lds_bytes
 }
 
 int lq_synchronize(void) {
+    LQ_API_GUARD;
     int rc = ensure_init();
     if (rc) return rc;
     LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
@@ -344,6 +351,7 @@ static hipEvent_t g_events[64];
 static bool g_event_made[64];
 
 int lq_event_record(int slot) {
+    LQ_API_GUARD;
     int rc = ensure_init();
     if (rc) return rc;
     if (slot < 0 || slot >= 64) {
@@ -359,6 +367,7 @@ int lq_event_record(int slot) {
 }
 
 int lq_event_elapsed_ms(int a, int b, float* ms) {
+    LQ_API_GUARD;
     if (a < 0 || a >= 64 || b < 0 || b >= 64 || !g_event_made[a] || !g_event_made[b] || !ms) {
         set_error("event slots not recorded");
         return LQ_ERR_INVALID_ARGUMENT;
@@ -369,6 +378,7 @@ int lq_event_elapsed_ms(int a, int b, float* ms) {
 }
 
 int lq_state_create(int num_qubits, int dtype, lq_state** out) {
+    LQ_API_GUARD;
     clear_error();
     if (!out) {
         set_error("out pointer is null");
@@ -409,6 +419,7 @@ int lq_state_create(int num_qubits, int dtype, lq_state** out) {
 }
 
 int lq_state_destroy(lq_state* st) {
+    LQ_API_GUARD;
     if (!st) return LQ_OK;
     if (st->d_amps) {
         (void)hipStreamSynchronize(g_stream);
@@ -420,6 +431,7 @@ int lq_state_destroy(lq_state* st) {
 }
 
 int lq_state_num_qubits(const lq_state* st, int* num_qubits) {
+    LQ_API_GUARD;
     int rc = check_state(st);
     if (rc) return rc;
     *num_qubits = st->n;
@@ -427,6 +439,7 @@ int lq_state_num_qubits(const lq_state* st, int* num_qubits) {
 }
 
 int lq_state_dtype(const lq_state* st, int* dtype) {
+    LQ_API_GUARD;
     int rc = check_state(st);
     if (rc) return rc;
     *dtype = st->dtype;
@@ -434,6 +447,7 @@ int lq_state_dtype(const lq_state* st, int* dtype) {
 }
 
 int lq_state_set_basis(lq_state* st, uint64_t index) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -445,6 +459,7 @@ int lq_state_set_basis(lq_state* st, uint64_t index) {
 }
 
 int lq_state_set_plus(lq_state* st) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -453,6 +468,7 @@ int lq_state_set_plus(lq_state* st) {
 }
 
 int lq_state_upload(lq_state* st, const double* amps, uint64_t len, int normalize) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -486,6 +502,7 @@ int lq_state_upload(lq_state* st, const double* amps, uint64_t len, int normaliz
 }
 
 int lq_state_download(const lq_state* st, double* amps, uint64_t len) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -506,6 +523,7 @@ int lq_state_download(const lq_state* st, double* amps, uint64_t len) {
 }
 
 int lq_state_copy(lq_state* dst, const lq_state* src) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(dst);
     if (rc) return rc;
@@ -522,6 +540,7 @@ int lq_state_copy(lq_state* dst, const lq_state* src) {
 }
 
 int lq_state_normalize(lq_state* st, double* norm_out) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -529,6 +548,7 @@ int lq_state_normalize(lq_state* st, double* norm_out) {
 }
 
 int lq_state_norm_sqr(const lq_state* st, double* out) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;
@@ -545,6 +565,7 @@ int lq_state_norm_sqr(const lq_state* st, double* out) {
 }
 
 int lq_state_inner_product(const lq_state* a, const lq_state* b, double out[2]) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(a);
     if (rc) return rc;
@@ -579,6 +600,7 @@ int lq_state_inner_product(const lq_state* a, const lq_state* b, double out[2]) 
 }
 
 int lq_state_probabilities(const lq_state* st, double* out, uint64_t len) {
+    LQ_API_GUARD;
     clear_error();
     int rc = check_state(st);
     if (rc) return rc;

@@ -36,11 +36,13 @@ static lq_gate mk_gate(int kind, int q0, int q1, double th) {
 using namespace lq;
 
 extern "C" int lq_set_gate_kernel_config(int unroll, int nontemporal) {
+    LQ_API_GUARD;
     clear_error();
     return set_gate_kernel_config(unroll, nontemporal);
 }
 
 extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t report_len) {
+    LQ_API_GUARD;
     clear_error();
     if (num_qubits < 8 || num_qubits > kMaxQubits || (dtype != LQ_C128 && dtype != LQ_C64)) {
         set_error("autotune needs 8 <= num_qubits <= %d and a valid dtype", kMaxQubits);
