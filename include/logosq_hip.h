@@ -211,7 +211,8 @@ int lq_execute(lq_state* st, const lq_gate* gates, int count, int mode);
  * contiguous; 0 keeps the default (12 / 13 / 5).  Used by the autotuner. */
 int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits);
 /* Per-gate kernel variant (no counterpart in the reference): work units per
- * wavefront (2, 4 or 8) and nontemporal global access on/off. */
+ * wavefront (1, 2, 4 or 8; default 1) and nontemporal global access on/off
+ * (default on). */
 int lq_set_gate_kernel_config(int unroll, int nontemporal);
 /* Autotuner (no counterpart in the reference): times the per-gate kernel
  * variants and the fused tile geometries on a num_qubits state of dtype

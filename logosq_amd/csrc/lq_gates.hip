@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(kThreads) k_dense(const double2* __restrict__ 
 // ---- host launchers -----------------------------------------------------------------
 // Variant knobs (autotuned): work units per wave (2, 4, 8) and nontemporal
 // global access.
-static int g_unroll = 2;
+static int g_unroll = 1;
 static bool g_nontemporal = true;
 
 // Fixed bits whose runs of equal value are shorter than this many bytes
@@ -506,16 +506,22 @@ static unsigned blocks_for_units(uint64_t units, int unroll) {
     return (unsigned)blocks;
 }
 
-// Calls F.template operator()<U, NT>() for the configured variant.
+// Calls F.template operator()<U, NT>() for the configured variant.  The
+// SWAP and 4x4 kernels ask for at least 2 units per wavefront (min_unroll):
+// with their two or four loads per lane they are faster with two units
+// than with one, which suits the pair and diagonal kernels.
 template <typename F>
-static void with_variant(F&& f) {
+static void with_variant(F&& f, int min_unroll = 1) {
+    const int u = g_unroll < min_unroll ? min_unroll : g_unroll;
     if (g_nontemporal) {
-        if (g_unroll == 2) f.template operator()<2, true>();
-        else if (g_unroll == 8) f.template operator()<8, true>();
+        if (u == 1) f.template operator()<1, true>();
+        else if (u == 2) f.template operator()<2, true>();
+        else if (u == 8) f.template operator()<8, true>();
         else f.template operator()<4, true>();
     } else {
-        if (g_unroll == 2) f.template operator()<2, false>();
-        else if (g_unroll == 8) f.template operator()<8, false>();
+        if (u == 1) f.template operator()<1, false>();
+        else if (u == 2) f.template operator()<2, false>();
+        else if (u == 8) f.template operator()<8, false>();
         else f.template operator()<4, false>();
     }
 }
@@ -590,8 +596,8 @@ int launch_diag(lq_state* st, uint64_t fixed_mask, int sel0, int sel1, const dou
 }
 
 int set_gate_kernel_config(int unroll, int nontemporal) {
-    if (unroll != 2 && unroll != 4 && unroll != 8) {
-        set_error("unroll must be 2, 4 or 8");
+    if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8) {
+        set_error("unroll must be 1, 2, 4 or 8");
         return LQ_ERR_INVALID_ARGUMENT;
     }
     g_unroll = unroll;
@@ -606,7 +612,7 @@ static void swap2_t(lq_state* st, int ba, int bb) {
     with_variant([&]<int U, bool NT>() {
         hipLaunchKernelGGL((k_swap2<V, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0, stream(),
                            psi, g.units, g.sk, g.lane_limit, 1ull << ba, 1ull << bb);
-    });
+    }, 2);
 }
 
 int launch_quad(lq_state* st, int ba, int bb, const double m[32]);
@@ -637,14 +643,14 @@ static void quad_t(lq_state* st, int ba, int bb, const double* md) {
         with_variant([&]<int U, bool NT>() {
             hipLaunchKernelGGL((k_quad<V, T, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
                                stream(), psi, g.units, g.sk, g.lane_limit, 1ull << ba, 1ull << bb, m);
-        });
+        }, 2);
     } else if (lo_a && lo_b) {
         // both among the lane bits (nothing skipped, so lane bit = amplitude bit)
         const Geometry g = geometry(st->n, 0, 0, st->amp_bytes);
         with_variant([&]<int U, bool NT>() {
             hipLaunchKernelGGL((k_quad_lo<V, T, 2, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
                                stream(), psi, g.units, g.sk, g.lane_limit, ba, bb, (uint64_t)0, m);
-        });
+        }, 2);
     } else {
         // one lane bit (below bit 6, so below the skipped one), one enumerated
         const int hb = lo_a ? bb : ba;
@@ -653,7 +659,7 @@ static void quad_t(lq_state* st, int ba, int bb, const double* md) {
             hipLaunchKernelGGL((k_quad_lo<V, T, 1, U, NT>), dim3(blocks_for_units(g.units, U)), dim3(kThreads), 0,
                                stream(), psi, g.units, g.sk, g.lane_limit, lo_a ? ba : -1, lo_b ? bb : -1,
                                1ull << hb, m);
-        });
+        }, 2);
     }
 }
 

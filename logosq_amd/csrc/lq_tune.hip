@@ -64,12 +64,12 @@ extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t repor
     std::vector<lq_gate> probe = {mk_gate(LQ_GATE_RX, n - 1, -1, 0.3), mk_gate(LQ_GATE_RX, n / 2, -1, 0.7),
                                   mk_gate(LQ_GATE_RX, 0, -1, 1.1), mk_gate(LQ_GATE_CNOT, n / 2, n / 2 + 1, 0),
                                   mk_gate(LQ_GATE_CNOT, 1, 2, 0), mk_gate(LQ_GATE_RZ, 3, -1, 0.2)};
-    const int unrolls[3] = {2, 4, 8};
+    const int unrolls[4] = {1, 2, 4, 8};
     float best = 1e30f;
     int best_u = 4, best_nt = 0;
     bool first = true;
     for (int nt = 0; nt < 2 && !rc; ++nt)
-        for (int ui = 0; ui < 3 && !rc; ++ui) {
+        for (int ui = 0; ui < 4 && !rc; ++ui) {
             set_gate_kernel_config(unrolls[ui], nt);
             float ms = 0.f;
             for (int rep_i = 0; rep_i < 2 && !rc; ++rep_i) {  // first round warms up

@@ -49,7 +49,7 @@ def test_every_gate_every_target(hip, n):
         assert err <= AMP_TOL, (g.kind, list(g.qubits), err)
 
 
-@pytest.mark.parametrize("unroll,nt", [(2, 0), (4, 1), (8, 0), (8, 1)])
+@pytest.mark.parametrize("unroll,nt", [(1, 0), (2, 0), (2, 1), (4, 1), (8, 0), (8, 1)])
 def test_gate_kernel_variants(hip, unroll, nt):
     """Every per-gate kernel variant the autotuner can pick gives the oracle's
     result (complex128 and complex64)."""
@@ -65,7 +65,7 @@ def test_gate_kernel_variants(hip, unroll, nt):
             assert np.max(np.abs(gpu_apply(psi, n, gates) - want)) <= AMP_TOL
             assert np.max(np.abs(gpu_apply(psi, n, gates, lq.C64) - want)) <= C64_TOL
     finally:
-        lq._lib.call("lq_set_gate_kernel_config", 2, 1)
+        lq._lib.call("lq_set_gate_kernel_config", 1, 1)
 
 
 @pytest.mark.parametrize("n", [7, 12])
