@@ -22,7 +22,7 @@
 
 namespace lq {
 
-constexpr int kExpBlocksMax = 2048;
+constexpr int kExpBlocksMax = 8192;
 constexpr int kExpThreads = 256;
 
 template <int NT>
