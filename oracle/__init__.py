@@ -46,7 +46,7 @@ def lib():
         pu8 = ctypes.POINTER(ctypes.c_uint8)
         sig = {
             "or_normalize": (d, [vp, u64]),
-            "or_inner_product": (None, [vp, vp, u64, pd]),
+            "or_inner_product": (None, [vp, vp, u64, ctypes.c_void_p]),
             "or_apply_gate": (i, [vp, i, pg]),
             "or_execute": (i, [vp, i, pg, i]),
             "or_matrix_gate": (None, [vp, i, vp, u64, vp]),
@@ -57,6 +57,7 @@ def lib():
             "or_pauli_term_expectation": (d, [vp, i, pu8, d]),
             "or_pauli_sum_expectation": (d, [vp, i, pu8, pd, i]),
             "or_param_shift": (None, [i, pg, i, pd, i, d, pu8, pd, i, pd]),
+            "or_param_shift_range": (None, [i, pg, i, pd, i, d, pu8, pd, i, i, i, pd]),
             "or_finite_difference": (None, [i, pg, i, pd, i, d, pu8, pd, i, pd]),
             "or_energy": (d, [i, pg, i, pd, pu8, pd, i]),
             "or_gd_minimize": (OptResult, [d, i, d, OBJ, GRD, vp, pd, i, pd]),
@@ -92,6 +93,15 @@ def normalize(v):
     v, p = _c(np.array(v, dtype=np.complex128, copy=True))
     norm = lib().or_normalize(p, len(v))
     return v, norm
+
+
+def inner_product(a, b):
+    """<a|b> (src/states/mod.rs:350-391), summed in index order."""
+    a, pa = _c(a)
+    b, pb = _c(b)
+    out = (ctypes.c_double * 2)()
+    lib().or_inner_product(pa, pb, len(a), out)
+    return complex(out[0], out[1])
 
 
 def apply_gate(v, n, g):
@@ -166,13 +176,30 @@ def energy(n, tmpl, params, codes, coeffs):
                            _pd(coeffs), len(coeffs))
 
 
-def param_shift(n, tmpl, params, codes, coeffs, shift=np.pi / 2):
+def param_shift(n, tmpl, params, codes, coeffs, shift=np.pi / 2, threads=1):
+    """ParameterShift::compute_gradient.  threads > 1 splits the components
+    into contiguous ranges evaluated in parallel (the C loop releases the
+    GIL and is re-entrant); every component is computed exactly as in the
+    serial loop."""
     arr = abi.gate_array(tmpl)
     params = np.ascontiguousarray(params, dtype=np.float64)
     codes, coeffs = _codes(codes, coeffs)
     g = np.zeros(len(params))
-    lib().or_param_shift(n, arr, len(tmpl), _pd(params), len(params), shift,
-                         codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _pd(coeffs), len(coeffs), _pd(g))
+    pc = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    if threads <= 1:
+        lib().or_param_shift(n, arr, len(tmpl), _pd(params), len(params), shift, pc, _pd(coeffs), len(coeffs),
+                             _pd(g))
+        return g
+    import concurrent.futures
+    P = len(params)
+    bounds = [(P * k // threads, P * (k + 1) // threads) for k in range(threads)]
+
+    def part(b):
+        lib().or_param_shift_range(n, arr, len(tmpl), _pd(params), P, shift, pc, _pd(coeffs), len(coeffs), b[0], b[1],
+                                   _pd(g))
+
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(part, bounds))
     return g
 
 

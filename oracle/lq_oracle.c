@@ -556,9 +556,16 @@ double or_energy(int n, const lq_gate* tmpl, int num_gates, const double* params
 
 void or_param_shift(int n, const lq_gate* tmpl, int num_gates, const double* params, int num_params, double shift,
                     const uint8_t* paulis, const double* coeffs, int num_terms, double* grad_out) {
-    /* src/optimization/gradient.rs:45-78 */
+    or_param_shift_range(n, tmpl, num_gates, params, num_params, shift, paulis, coeffs, num_terms, 0, num_params,
+                         grad_out);
+}
+
+void or_param_shift_range(int n, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
+                          double shift, const uint8_t* paulis, const double* coeffs, int num_terms, int first,
+                          int last, double* grad_out) {
+    /* src/optimization/gradient.rs:45-78 (the loop over i, for i in [first, last)) */
     double* p = (double*)malloc((size_t)num_params * sizeof(double));
-    for (int i = 0; i < num_params; ++i) {
+    for (int i = first; i < last; ++i) {
         memcpy(p, params, (size_t)num_params * sizeof(double));
         p[i] += shift;
         double ep = or_energy(n, tmpl, num_gates, p, paulis, coeffs, num_terms);

@@ -95,6 +95,11 @@ void or_run_template(orc* v, int n, const lq_gate* tmpl, int num_gates, const do
 /* ParameterShift::compute_gradient (src/optimization/gradient.rs:45-78). */
 void or_param_shift(int n, const lq_gate* tmpl, int num_gates, const double* params, int num_params, double shift,
                     const uint8_t* paulis, const double* coeffs, int num_terms, double* grad_out);
+/* The same loop over components first .. last-1 only (grad_out[i] for those
+ * i); re-entrant, so disjoint ranges may run in parallel threads. */
+void or_param_shift_range(int n, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
+                          double shift, const uint8_t* paulis, const double* coeffs, int num_terms, int first,
+                          int last, double* grad_out);
 /* FiniteDifference::compute_gradient (src/optimization/gradient.rs:99-125). */
 void or_finite_difference(int n, const lq_gate* tmpl, int num_gates, const double* params, int num_params,
                           double eps, const uint8_t* paulis, const double* coeffs, int num_terms,

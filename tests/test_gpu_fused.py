@@ -72,6 +72,23 @@ def test_fused_brickwork_22(hip):
     assert np.max(np.abs(per - want)) <= 1e-12
 
 
+def test_fused_brickwork_22_c64(hip):
+    """The same circuit in complex64 (fused and per-gate) against the c128
+    oracle: amplitudes <= 2e-6 (f32 storage), relative 2-norm error <= 1e-5."""
+    say("fused brickwork 22 c64")
+    n = 22
+    d = lq.algorithms.brickwork_circuit(n, 20, 2).descriptors()
+    psi = oracle.zero_state(n)
+    want = oracle.execute(psi, n, d)
+    for mode in (abi.EXEC_FUSED, abi.EXEC_PER_GATE):
+        got = run(psi, n, d, lq.C64, mode)
+        err = float(np.max(np.abs(got - want)))
+        rel = float(np.linalg.norm(got - want))
+        say("  c64 mode %d: max |delta| %.3e, relative 2-norm %.3e" % (mode, err, rel))
+        assert err <= 2e-6
+        assert rel <= 1e-5
+
+
 def test_fused_qft_12(hip):
     say("fused qft12")
     n = 12
