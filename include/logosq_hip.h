@@ -257,6 +257,22 @@ int lq_noise_depolarizing(lq_state* st, const int* qubits, int count, double p, 
  * qubit q.  convention is LQ_PAULI_REFERENCE or LQ_PAULI_PHYSICAL. */
 int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
                         int convention, double* out);
+/* lq_expect_pauli_sum without allocating (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and result as lq_expect_pauli_sum, but the term tables
+ * and reduction partials live in a static 4 MiB device array that each GPU
+ * holds from the library's load to the end of the process, and no pinned
+ * host memory is taken.  A sum whose tables and partials need more than
+ * 4 MiB on this state is rejected with LQ_ERR_INVALID_ARGUMENT before any
+ * device work (the XYZ ring on 30 qubits needs about 2.1 MiB with LDS tiles
+ * off).  Every call on a GPU shares that GPU's array; nothing in it has to
+ * start at zero (each call writes all it reads).  Calls are serialised by
+ * the library lock and wait for their kernels, so they do not overlap
+ * within a process; device work of two calls that did overlap on one GPU
+ * would race on the array. */
+int lq_expect_pauli_sum_noalloc(const lq_state* st, const uint8_t* paulis, const double* coeffs, int num_terms,
+                                int convention, double* out);
 /* How flip groups (terms sharing their X/Y bits) are evaluated (no
  * reference counterpart): LQ_EXPECT_TILES_AUTO (default) reads a state over
  * 64 MiB once per tile layout, evaluating every flip group whose bits lie in

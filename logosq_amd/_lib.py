@@ -65,6 +65,7 @@ SIGNATURES = {
     "lq_autotune": [c_int, c_int, ctypes.c_char_p, ctypes.c_size_t],
     "lq_plan_fused": [c_int, c_int, P_gate, c_int, c_int, c_int, P_int, P_int, P_int],
     "lq_expect_pauli_sum": [c_void_p, P_u8, P_double, c_int, c_int, P_double],
+    "lq_expect_pauli_sum_noalloc": [c_void_p, P_u8, P_double, c_int, c_int, P_double],
     "lq_param_shift_gradient": [c_int, c_int, P_gate, c_int, P_double, c_int, c_double, P_u8, P_double, c_int,
                                 c_int, c_int, P_double, P_double],
     "lq_adjoint_gradient": [c_int, c_int, P_gate, c_int, P_double, c_int, P_u8, P_double, c_int, c_int, P_double,

@@ -483,8 +483,16 @@ int compile_pauli_sum(int n, const uint8_t* paulis, const double* coeffs, int nu
     return LQ_OK;
 }
 
+// Device memory an expectation works in when the caller supplies it (the
+// no-allocation entry point): its base and size.  Null: the library's
+// growable scratch and pinned staging.
+struct ExpWorkspace {
+    void* base;
+    size_t bytes;
+};
+
 template <typename V>
-static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
+static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const ExpWorkspace* ws) {
     const int ng = (int)ps.groups.size();
     if (ng == 0) {
         *out = 0.0;
@@ -546,17 +554,32 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
         lay_bytes += al(l.groups.size() * sizeof(ExpTileGroup)) + al(l.terms.size() * sizeof(ExpTileTerm));
     }
     size_t part_count = launches * nb + tile_parts;
-    char* base = (char*)scratch(term_pad + lay_bytes + (part_count + 8) * sizeof(double));
-    if (!base) {
-        set_error("scratch allocation failed");
-        return LQ_ERR_OUT_OF_MEMORY;
+    const size_t need = term_pad + lay_bytes + (part_count + 8) * sizeof(double);
+    char* base;
+    double* h;
+    std::vector<double> pageable;  // the tables' host copy when no pinned staging may be allocated
+    if (ws) {
+        if (need > ws->bytes) {
+            set_error("this Pauli sum needs %zu bytes of workspace on a %d-qubit state; the static workspace holds %zu",
+                      need, st->n, ws->bytes);
+            return LQ_ERR_INVALID_ARGUMENT;
+        }
+        base = (char*)ws->base;
+        pageable.resize((term_pad + lay_bytes) / sizeof(double) + 16);
+        h = pageable.data();
+    } else {
+        base = (char*)scratch(need);
+        if (!base) {
+            set_error("scratch allocation failed");
+            return LQ_ERR_OUT_OF_MEMORY;
+        }
+        h = host_staging((term_pad + lay_bytes) / sizeof(double) + 16);
+        if (!h) return LQ_ERR_OUT_OF_MEMORY;
     }
     PauliTermDev* d_terms = (PauliTermDev*)base;
     char* d_lay = base + term_pad;
     double* d_part = (double*)(base + term_pad + lay_bytes);
     double* d_out = d_part + part_count;
-    double* h = host_staging((term_pad + lay_bytes) / sizeof(double) + 16);
-    if (!h) return LQ_ERR_OUT_OF_MEMORY;
     std::memcpy(h, dev.data(), term_bytes);
     for (size_t li = 0; li < lays.size(); ++li) {
         char* hp = (char*)h + term_pad + lay_off[li];
@@ -613,17 +636,54 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out) {
     return LQ_OK;
 }
 
-int expect_pauli_sum(const lq_state* st, const PauliSum& ps, double* out) {
+static int expect_ws(const lq_state* st, const PauliSum& ps, double* out, const ExpWorkspace* ws) {
     if (ps.n != st->n) {
         // "Pauli term must have operator for each qubit" (observable.rs:97-101)
         set_error("Pauli term must have operator for each qubit (%d terms' qubits vs state %d)", ps.n, st->n);
         return LQ_ERR_DIMENSION_MISMATCH;
     }
-    if (st->dtype == LQ_C128) return expect_t<double2>(st, ps, out);
-    return expect_t<float2>(st, ps, out);
+    if (st->dtype == LQ_C128) return expect_t<double2>(st, ps, out, ws);
+    return expect_t<float2>(st, ps, out, ws);
 }
 
+int expect_pauli_sum(const lq_state* st, const PauliSum& ps, double* out) { return expect_ws(st, ps, out, nullptr); }
+
 }  // namespace lq
+
+// Workspace of lq_expect_pauli_sum_noalloc: term tables, tile layouts and
+// per-block partials.  Every GPU the library runs on holds its own copy until
+// the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (g_noalloc_ptr), never
+// through this name.
+static constexpr size_t kNoAllocBytes = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+void* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// The workspace address on the current device, looked up on the first call
+// made on that device (under the library lock).
+int noalloc_workspace(void** out) {
+    int dev = 0;
+    LQ_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) {
+        lq::set_error("device %d is beyond the %d devices the static workspace table holds", dev, kNoAllocDevices);
+        return LQ_ERR_UNSUPPORTED;
+    }
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || !p) {
+            lq::set_error("hipGetSymbolAddress of the static workspace failed: %s", hipGetErrorString(e));
+            return LQ_ERR_DEVICE;
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return LQ_OK;
+}
+}  // namespace
 
 using namespace lq;
 
@@ -675,4 +735,23 @@ extern "C" int lq_expect_pauli_sum(const lq_state* st, const uint8_t* paulis, co
     rc = compile_pauli_sum(st->n, paulis, coeffs, num_terms, convention, &ps);
     if (rc) return rc;
     return expect_pauli_sum(st, ps, out);
+}
+
+extern "C" int lq_expect_pauli_sum_noalloc(const lq_state* st, const uint8_t* paulis, const double* coeffs,
+                                           int num_terms, int convention, double* out) {
+    LQ_API_GUARD;
+    clear_error();
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (!out) {
+        set_error("out is null");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    PauliSum ps;
+    rc = compile_pauli_sum(st->n, paulis, coeffs, num_terms, convention, &ps);
+    if (rc) return rc;
+    ExpWorkspace ws{nullptr, kNoAllocBytes};
+    rc = noalloc_workspace(&ws.base);
+    if (rc) return rc;
+    return expect_ws(st, ps, out, &ws);
 }
