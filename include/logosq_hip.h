@@ -285,9 +285,25 @@ int lq_expect_pauli_sum_noalloc(const lq_state* st, const uint8_t* paulis, const
 #define LQ_EXPECT_TILES_ON 2
 int lq_set_expect_tiles(int mode);
 /* The reads of the state an expectation of this sum makes in the current
- * mode: tile layouts, flip groups read on their own, diagonal groups. */
+ * modes: tile layouts, flip groups read on their own, diagonal groups read
+ * on their own.  With run-time specialisation on (lq_set_jit) and tiles
+ * wanted for the size, the layouts are those of the specialised kernels
+ * (128 KiB tiles, the diagonal group inside the first layout), used once
+ * they compile. */
 int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
                    int convention, int* tile_layouts, int* pair_groups, int* diag_groups);
+/* Compile the specialised expectation kernels of this sum on a num_qubits
+ * state ahead of time (no device needed; no reference counterpart), so that
+ * a timed call does not compile.  kernels: layouts compiled (0 when the
+ * modes do not use them at this size). */
+int lq_expect_jit_compile(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
+                          int convention, int* kernels);
+/* The specialised plan of this sum as JSON (introspection, no reference
+ * counterpart): per layout its tile geometry, coefficient table and kernel
+ * text, and the flip groups left to the pair kernel; "{}" when the modes do
+ * not use it at this size.  needed (nullable) gets the buffer size. */
+int lq_expect_jit_describe(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs, int num_terms,
+                           int convention, char* out, size_t out_len, size_t* needed);
 
 /* ---- Gradients (src/optimization/gradient.rs) --------------------------- */
 /* ParameterShift::compute_gradient (gradient.rs:45-78) for an ansatz given as

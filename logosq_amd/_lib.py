@@ -79,6 +79,9 @@ SIGNATURES = {
     "lq_set_jit": [c_int],
     "lq_set_expect_tiles": [c_int],
     "lq_expect_plan": [c_int, c_int, P_u8, P_double, c_int, c_int, P_int, P_int, P_int],
+    "lq_expect_jit_compile": [c_int, c_int, P_u8, P_double, c_int, c_int, P_int],
+    "lq_expect_jit_describe": [c_int, c_int, P_u8, P_double, c_int, c_int, ctypes.c_char_p, ctypes.c_size_t,
+                               ctypes.POINTER(ctypes.c_size_t)],
     "lq_jit_stats": [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                      ctypes.POINTER(ctypes.c_longlong), P_double],
     "lq_jit_compile_plan": [c_int, c_int, P_gate, c_int, P_int],

@@ -17,6 +17,7 @@
 #include <map>
 #include <type_traits>
 
+#include "lq_expect_jit.h"
 #include "lq_host.h"
 #include "lq_tile_ops.h"
 
@@ -518,13 +519,25 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     std::vector<PauliTermDev> dev = ps.terms;
     int diag_first = -1, diag_static = 0, diag_count = 0;
     size_t launches = 0;
-    // flip groups: from LDS tiles where they fit a layout, else pair kernels
+    // flip groups: from LDS tiles where they fit a layout, else pair kernels;
+    // the run-time specialised layouts when the policy allows and all compile
     std::vector<ExpLayout> lays;
     std::vector<int> rest;
-    plan_expect_tiles(st->n, st->dtype, ps, lays, rest);
+    XPlan xp;
+    // (not for a caller-supplied workspace: compiling and loading a kernel allocate)
+    const bool use_x = !ws && expect_jit_wanted(st->n, st->dtype) && plan_expect_jit(st->n, st->dtype, ps, xp) &&
+                       prepare_expect_jit(xp, false, true);
+    int x_gx = 0;
+    if (use_x) {
+        rest = xp.rest;
+        x_gx = expect_jit_blocks(xp, 1);
+    } else {
+        plan_expect_tiles(st->n, st->dtype, ps, lays, rest);
+    }
+    const bool diag_in_tiles = use_x && xp.diag_in;
     std::vector<char> paired(ng, 0);
     for (int g : rest) paired[g] = 1;
-    size_t tile_parts = 0;
+    size_t tile_parts = use_x ? (size_t)x_gx * xp.layouts.size() : 0;
     std::vector<unsigned> lay_blocks;
     for (const ExpLayout& l : lays) {
         lay_blocks.push_back(expect_tile_blocks(l.geo, 1));
@@ -535,6 +548,7 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
             if (paired[&grp - ps.groups.data()]) ++launches;
             continue;
         }
+        if (diag_in_tiles) continue;
         std::vector<PauliTermDev> st_, dy_;
         for (int t = grp.first; t < grp.first + grp.count; ++t)
             ((ps.terms[t].zy & rmask) == 0 ? st_ : dy_).push_back(ps.terms[t]);
@@ -553,6 +567,8 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
         lay_off.push_back(lay_bytes);
         lay_bytes += al(l.groups.size() * sizeof(ExpTileGroup)) + al(l.terms.size() * sizeof(ExpTileTerm));
     }
+    const size_t x_coef_off = lay_bytes;  // the specialised layouts' coefficient table
+    if (use_x) lay_bytes += al(expect_jit_coef_count(xp) * sizeof(double));
     size_t part_count = launches * nb + tile_parts;
     const size_t need = term_pad + lay_bytes + (part_count + 8) * sizeof(double);
     char* base;
@@ -581,6 +597,7 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     double* d_part = (double*)(base + term_pad + lay_bytes);
     double* d_out = d_part + part_count;
     std::memcpy(h, dev.data(), term_bytes);
+    if (use_x) expect_jit_coefs(xp, (double*)((char*)h + term_pad + x_coef_off));
     for (size_t li = 0; li < lays.size(); ++li) {
         char* hp = (char*)h + term_pad + lay_off[li];
         const ExpLayout& l = lays[li];
@@ -590,6 +607,12 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     }
     LQ_HIP_CHECK(hipMemcpyAsync(d_terms, h, term_pad + lay_bytes, hipMemcpyHostToDevice, stream()));
     int written = 0;
+    if (use_x) {
+        int rc = launch_expect_jit(xp, st->d_amps, (const double*)(d_lay + x_coef_off), 0, 1, 0, x_gx,
+                                   d_part + written);
+        if (rc) return rc;
+        written += x_gx * (int)xp.layouts.size();
+    }
     for (size_t li = 0; li < lays.size(); ++li) {
         const ExpLayout& l = lays[li];
         const char* dp = d_lay + lay_off[li];
@@ -602,6 +625,7 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     }
     for (int g = 0; g < ng; ++g) {
         const PauliGroup& grp = ps.groups[g];
+        if (grp.x == 0 && diag_in_tiles) continue;
         if (grp.x == 0) {
             // launch 0: the static terms and the first dynamic ones
             const int nd_all = diag_count - diag_static;
@@ -699,14 +723,45 @@ extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, 
     PauliSum ps;
     int rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
     if (rc) return rc;
+    int diag = 0;
+    for (const PauliGroup& g : ps.groups) diag += g.x == 0;
+    XPlan xp;
+    if (expect_jit_wanted(num_qubits, dtype) && plan_expect_jit(num_qubits, dtype, ps, xp)) {
+        // the run-time specialised layouts (used once their kernels compile)
+        if (tile_layouts) *tile_layouts = (int)xp.layouts.size();
+        if (pair_groups) *pair_groups = (int)xp.rest.size();
+        if (diag_groups) *diag_groups = xp.diag_in ? 0 : diag;
+        return LQ_OK;
+    }
     std::vector<ExpLayout> lays;
     std::vector<int> rest;
     plan_expect_tiles(num_qubits, dtype, ps, lays, rest);
-    int diag = 0;
-    for (const PauliGroup& g : ps.groups) diag += g.x == 0;
     if (tile_layouts) *tile_layouts = (int)lays.size();
     if (pair_groups) *pair_groups = (int)rest.size();
     if (diag_groups) *diag_groups = diag;
+    return LQ_OK;
+}
+
+extern "C" int lq_expect_jit_compile(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs,
+                                     int num_terms, int convention, int* kernels) {
+    LQ_API_GUARD;
+    clear_error();
+    if (num_qubits < 1 || num_qubits > kMaxQubits || num_terms < 0 || (num_terms > 0 && (!paulis || !coeffs)) ||
+        (dtype != LQ_C128 && dtype != LQ_C64)) {
+        set_error("invalid arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    PauliSum ps;
+    int rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
+    if (rc) return rc;
+    if (kernels) *kernels = 0;
+    XPlan xp;
+    if (!expect_jit_wanted(num_qubits, dtype) || !plan_expect_jit(num_qubits, dtype, ps, xp)) return LQ_OK;
+    if (!prepare_expect_jit(xp, true, false)) {
+        set_error("hipRTC failed on an expectation kernel (LQ_JIT_LOG=1 prints the log)");
+        return LQ_ERR_DEVICE;
+    }
+    if (kernels) *kernels = (int)xp.layouts.size();
     return LQ_OK;
 }
 
@@ -754,4 +809,28 @@ extern "C" int lq_expect_pauli_sum_noalloc(const lq_state* st, const uint8_t* pa
     rc = noalloc_workspace(&ws.base);
     if (rc) return rc;
     return expect_ws(st, ps, out, &ws);
+}
+
+extern "C" int lq_expect_jit_describe(int num_qubits, int dtype, const uint8_t* paulis, const double* coeffs,
+                                      int num_terms, int convention, char* out, size_t out_len, size_t* needed) {
+    LQ_API_GUARD;
+    clear_error();
+    if (num_qubits < 1 || num_qubits > kMaxQubits || num_terms < 0 || (num_terms > 0 && (!paulis || !coeffs)) ||
+        (dtype != LQ_C128 && dtype != LQ_C64)) {
+        set_error("invalid arguments");
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
+    PauliSum ps;
+    int rc = compile_pauli_sum(num_qubits, paulis, coeffs, num_terms, convention, &ps);
+    if (rc) return rc;
+    XPlan xp;
+    std::string js = "{}";
+    if (expect_jit_wanted(num_qubits, dtype) && plan_expect_jit(num_qubits, dtype, ps, xp)) js = describe_expect_jit(xp);
+    if (needed) *needed = js.size() + 1;
+    if (out && out_len > 0) {
+        const size_t k = std::min(out_len - 1, js.size());
+        std::memcpy(out, js.data(), k);
+        out[k] = 0;
+    }
+    return LQ_OK;
 }

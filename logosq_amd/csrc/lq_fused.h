@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "lq_host.h"
@@ -64,5 +65,11 @@ void jit_prepare(Plan& plan, int dtype, uint64_t ninst);
 int jit_warm(int n, int dtype, const lq_gate* gates, int count);
 int launch_jit(const JitKernel* k, void* psi, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
                uint64_t inst_stride, uint64_t ops_stride, unsigned grid);
+// The run-time specialisation policy (LQ_JIT_*), shared by the expectation
+// kernels; hipRTC compile of one kernel text with the library's options
+// (thread-safe); and adding to the statistics (compiles, failures, seconds).
+int jit_mode();
+bool jit_compile_text(const std::string& src, std::vector<char>& code);
+void jit_count(int compiled, int failed, double seconds);
 
 }  // namespace lq

@@ -311,6 +311,16 @@ static bool jit_compile(const std::string& src, std::vector<char>& code) {
     return ok;
 }
 
+int jit_mode() { return g_jit_mode; }
+
+bool jit_compile_text(const std::string& src, std::vector<char>& code) { return jit_compile(src, code); }
+
+void jit_count(int compiled, int failed, double seconds) {
+    g_jit_compiled += compiled;
+    g_jit_failed += failed;
+    g_jit_seconds += seconds;
+}
+
 // Compile the given entries in parallel (at most 16 threads).
 static void compile_all(const std::vector<std::pair<const std::string*, JitKernel*>>& todo) {
     if (todo.empty()) return;

@@ -1,15 +1,18 @@
 """The expectation planner (host code, no GPU): which flip groups are read
-from LDS tile layouts, which alone, in each mode.  A layout holds the 5
-lowest bits and 6 (complex128) / 7 (complex64) more, so a ring's bonds pack
-about 6 to a layout; strings whose flip bits span more than a tile are read
-alone."""
+from LDS tile layouts, which alone, in each mode.  Generic tile kernels
+(run-time specialisation off): a layout holds the 5 lowest bits and 6
+(complex128) / 7 (complex64) more, so a ring's bonds pack about 6 to a
+layout.  Specialised kernels (the default): 128 KiB tiles whose low bits may
+shrink to one 128-byte line, the diagonal group inside the first layout, so
+the 30-qubit ring takes three reads (complex64: four).  Strings whose flip bits span more than
+a tile are read alone in both."""
 import ctypes
 
 import numpy as np
 import pytest
 
 import logosq_amd as lq
-from logosq_amd import abi, algorithms
+from logosq_amd import abi, algorithms, jit
 from tests import modes
 
 
@@ -27,8 +30,21 @@ def plan(n, codes, coeffs, dtype=abi.LQ_C128):
 def tiles_mode():
     lq.load()
     lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_AUTO)
+    jit.set_jit(jit.OFF)
     yield lambda m: lq._lib.call("lq_set_expect_tiles", m)
     modes.restore_defaults()
+
+
+def test_ring_30_specialised_three_reads(tiles_mode):
+    jit.set_jit(jit.REPEAT)
+    codes, coeffs = algorithms.heisenberg_codes(30, 0.7, -0.4, 0.9)
+    assert plan(30, codes, coeffs) == (3, 0, 0)
+    assert plan(30, codes, coeffs, abi.LQ_C64) == (4, 0, 0)
+    assert plan(20, codes[:0], coeffs[:0]) == (0, 0, 0)
+    small = algorithms.heisenberg_codes(20, 0.7, -0.4, 0.9)
+    assert plan(20, *small) == (0, 20, 1)  # 16 MiB: cache-resident, one read per group
+    tiles_mode(abi.EXPECT_TILES_OFF)
+    assert plan(30, codes, coeffs) == (0, 30, 1)
 
 
 def test_ring_30_auto_uses_tiles(tiles_mode):
@@ -59,6 +75,9 @@ def test_modes_and_wide_strings(tiles_mode):
     assert plan(n, codes, coeffs) == (0, n + 1, 1)
     tiles_mode(abi.EXPECT_TILES_ON)
     assert plan(7, *algorithms.heisenberg_codes(7, 1, 1, 1)) == (0, 7, 1)  # under 8 qubits: pair kernel
+    jit.set_jit(jit.REPEAT)  # specialised: the diagonal group joins the first layout
+    lay, pair, diag = plan(n, codes, coeffs)
+    assert lay >= 1 and pair == 1 and diag == 0
 
 
 def test_mode_rejects_unknown(tiles_mode):
