@@ -422,10 +422,43 @@ def cpu_baseline(nq, ngates, seed, depth):
         lib.or_apply_gate(p, nq, ctypes.byref(g))
     dt = time.perf_counter() - t0
     names = {abi.RX: "RX", abi.RY: "RY", abi.RZ: "RZ", abi.CNOT: "CNOT"}
-    return {"value": total_bytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+    return {"value": total_bytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port", "host": host_info(),
             "sample": "%d gates of layer 0 (%s) at %d qubits c128, oracle gate loops single-threaded as the "
                       "reference's, %.1f s" % (len(sample), ",".join(names.get(g.kind, str(g.kind)) for g in sample),
                                               nq, dt)}
+
+
+def host_info():
+    """The GPU box's host: logical CPUs, the CPUs this process may use, and
+    the CPU model (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)), "cpu_model": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not info["cpu_model"] and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    return info
+
+
+def copy_peak(lq, st, n, dtype, c, reps=3):
+    """Measured device copy rate: lq_state_copy of the whole 2^n-amplitude
+    state into a second state (the library's streaming copy kernel), read +
+    write bytes over the fastest of `reps` HIP-event times."""
+    dst = lq.State(n, dtype)
+    fn = lambda: lq._lib.call("lq_state_copy", dst.handle, st.handle)
+    fn()
+    t = min(event_time(lq, fn) for _ in range(reps))
+    dst = None
+    return 2 * c * (1 << n) / t / 1e9
 
 
 def run_circuit(lq, st, arr, count, mode, steps, warmup, label):
@@ -450,10 +483,12 @@ def expectation_bench(lq, st, n, c, reps=3):
     """Pauli-sum expectation on the circuit's output state: the periodic XYZ
     ring, 3n terms with J = (0.7, -0.4, 0.9).  The XX and YY terms of a bond
     share one flip mask and every ZZ term is diagonal: n + 1 groups.  The
-    flip groups are evaluated from LDS tiles, a few layouts each reading the
-    state once; `reads` counts the state reads (layouts + groups read alone +
-    the diagonal group), `bytes` = reads * c * N.  HIP-event time of the
-    whole lq_expect_pauli_sum call (fastest of `reps`)."""
+    groups are evaluated from LDS tiles by kernels specialised at run time
+    (compiled before timing), a few layouts each reading the state once, the
+    diagonal group inside the first; `reads` counts the state reads (layouts
+    + groups read alone + diagonal groups read alone), `bytes` = reads * c *
+    N.  HIP-event time of the whole lq_expect_pauli_sum call (fastest of
+    `reps`); beside it the generic tile kernels and one read per group."""
     import numpy as np
     codes, coeffs = lq.algorithms.heisenberg_codes(n, 0.7, -0.4, 0.9)
     groups = len({tuple(((row == 1) | (row == 2)).tolist()) for row in codes})
@@ -461,6 +496,8 @@ def expectation_bench(lq, st, n, c, reps=3):
     cp = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
     fp = coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     dtype = 0 if c == 16 else 1
+    kernels = ctypes.c_int()
+    lq._lib.call("lq_expect_jit_compile", n, dtype, cp, fp, len(coeffs), 0, ctypes.byref(kernels))
     lay, pair, diag = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lq._lib.call("lq_expect_plan", n, dtype, cp, fp, len(coeffs), 0, ctypes.byref(lay), ctypes.byref(pair),
                  ctypes.byref(diag))
@@ -468,6 +505,14 @@ def expectation_bench(lq, st, n, c, reps=3):
     fn = lambda: lq._lib.call("lq_expect_pauli_sum", st.handle, cp, fp, len(coeffs), 0, ctypes.byref(e))
     fn()
     t = min(event_time(lq, fn) for _ in range(reps))
+    e_spec = e.value
+    lq._lib.call("lq_set_jit", 0)  # the generic tile kernels, for comparison
+    try:
+        fn()
+        t_tiles = min(event_time(lq, fn) for _ in range(reps))
+        e_tiles = e.value
+    finally:
+        lq._lib.call("lq_set_jit", 1)
     lq._lib.call("lq_set_expect_tiles", 0)  # one read per group, for comparison
     try:
         fn()
@@ -475,12 +520,13 @@ def expectation_bench(lq, st, n, c, reps=3):
     finally:
         lq._lib.call("lq_set_expect_tiles", 1)
     gbs = reads * c * (1 << n) / t / 1e9
-    return {"workload": "periodic XYZ ring, %d terms, %d groups (%d flip groups in %d tile layouts, %d read "
-                        "alone, %d diagonal)" % (len(coeffs), groups, groups - diag.value - pair.value, lay.value,
-                                                 pair.value, diag.value),
+    return {"workload": "periodic XYZ ring, %d terms, %d groups: %d tile layouts (specialised kernels: %d), %d "
+                        "flip groups read alone, %d diagonal groups read alone"
+                        % (len(coeffs), groups, lay.value, kernels.value, pair.value, diag.value),
             "ms": round(t * 1e3, 3), "reads": reads, "bytes": reads * c * (1 << n), "GBps": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "ms_one_read_per_group": round(t_groups * 1e3, 3),
-            "energy": e.value}
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "ms_generic_tile_kernels": round(t_tiles * 1e3, 3),
+            "ms_one_read_per_group": round(t_groups * 1e3, 3), "energy": e_spec,
+            "energy_diff_generic": abs(e_spec - e_tiles), "energy_diff_one_read_per_group": abs(e_spec - e.value)}
 
 
 def c64_error(lq, n, depth, seed):
@@ -549,6 +595,8 @@ def main():
           file=sys.stderr, flush=True)
     times = run_circuit(lq, st, arr, len(descs), mode, a.steps, a.warmup, a.mode)
     norm2 = st.norm_sqr()
+    copy_gbs = copy_peak(lq, st, a.qubits, dtype, c)
+    print("bench: device copy %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
     expect = expectation_bench(lq, st, a.qubits, c)
     print("bench: expectation %.1f GB/s" % expect["GBps"], file=sys.stderr, flush=True)
     step_s = sum(times) / len(times)
@@ -650,8 +698,13 @@ def main():
                    "expectation": expect},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.mode),
-                     "kernel": "%s, %d B per launch, avg %.3f ms" % (kname, bpl, avg_s * 1e3)},
+                     "kernel": "%s, %d B per launch, avg %.3f ms" % (kname, bpl, avg_s * 1e3),
+                     "measured_peak": round(copy_gbs, 1),
+                     "measured_peak_kernel": "lq_state_copy of the %d-qubit state (k_copy16: one 16-byte unit per "
+                                             "thread, nontemporal, read + write bytes)" % a.qubits,
+                     "frac_vs_measured": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
+        "host": host_info(),
         "workloads": extra,
     }
     print(json.dumps(out), flush=True)

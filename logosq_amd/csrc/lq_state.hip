@@ -155,12 +155,18 @@ __global__ void __launch_bounds__(1024) k_final_sum(const double* __restrict__ p
 
 // a /= norm when norm > 1e-10 (src/states/mod.rs:101-103).  The division is
 // done in f64 for both amplitude types.
+// Streaming kernels take one element per thread and as many blocks as the
+// state needs: blocks are dispatched in address order, so the waves in flight
+// at any moment cover one contiguous window of memory (few pages to
+// translate, whole DRAM rows).  On MI355X a 2 x 16 GiB copy runs at 6.6 TB/s
+// this way, 4.6-5.3 TB/s as a grid-stride loop and 4.9 TB/s through
+// hipMemcpyAsync.
 template <typename V>
-__global__ void k_scale(V* __restrict__ a, uint64_t dim, const double* __restrict__ norm_ptr) {
-    double norm = norm_ptr[1];
+__global__ void __launch_bounds__(256) k_scale(V* __restrict__ a, uint64_t dim, const double* __restrict__ norm_ptr) {
+    const double norm = norm_ptr[1];
     if (!(norm > 1e-10)) return;
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < dim) {
         V x = a[i];
         x.x = (decltype(x.x))((double)x.x / norm);
         x.y = (decltype(x.y))((double)x.y / norm);
@@ -169,15 +175,27 @@ __global__ void k_scale(V* __restrict__ a, uint64_t dim, const double* __restric
 }
 
 template <typename V>
-__global__ void k_fill(V* __restrict__ a, uint64_t dim, double value, uint64_t one_index) {
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+__global__ void __launch_bounds__(256) k_fill(V* __restrict__ a, uint64_t dim, double value, uint64_t one_index) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < dim) {
         V x;
         x.x = (decltype(x.x))(i == one_index ? 1.0 : value);
         x.y = 0;
         a[i] = x;
     }
 }
+
+typedef double copy_unit __attribute__((ext_vector_type(2)));
+
+// Bit copy in 16-byte units (two complex64 or one complex128 amplitude).
+__global__ void __launch_bounds__(256) k_copy16(const copy_unit* __restrict__ src, copy_unit* __restrict__ dst,
+                                                uint64_t units) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < units) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+// Blocks of 256 threads covering `count` elements, one per thread.
+static unsigned blocks_one_per_thread(uint64_t count) { return (unsigned)((count + 255) / 256); }
 
 template <typename V>
 __global__ void __launch_bounds__(kRedThreads) k_inner(const V* __restrict__ a, const V* __restrict__ b,
@@ -282,7 +300,7 @@ int normalize_device(lq_state* st, double* norm_out) {
     }
     int rc = reduce_norm_sqr(st, d);
     if (rc) return rc;
-    unsigned g = grid_for(st->dim, 256, 16384);
+    const unsigned g = blocks_one_per_thread(st->dim);
     if (st->dtype == LQ_C128)
         hipLaunchKernelGGL(k_scale<double2>, dim3(g), dim3(256), 0, g_stream, (double2*)st->d_amps, st->dim,
                            (const double*)d);
@@ -301,7 +319,7 @@ int normalize_device(lq_state* st, double* norm_out) {
 }
 
 static int fill_state(lq_state* st, double value, uint64_t one_index) {
-    unsigned g = grid_for(st->dim, 256, 16384);
+    const unsigned g = blocks_one_per_thread(st->dim);
     if (st->dtype == LQ_C128)
         hipLaunchKernelGGL(k_fill<double2>, dim3(g), dim3(256), 0, g_stream, (double2*)st->d_amps, st->dim, value,
                            one_index);
@@ -534,8 +552,14 @@ int lq_state_copy(lq_state* dst, const lq_state* src) {
                   (unsigned long long)src->dim);
         return LQ_ERR_DIMENSION_MISMATCH;
     }
-    LQ_HIP_CHECK(hipMemcpyAsync(dst->d_amps, src->d_amps, src->dim * src->amp_bytes, hipMemcpyDeviceToDevice,
-                                g_stream));
+    const uint64_t bytes = src->dim * src->amp_bytes;
+    if (bytes % 16 == 0) {
+        hipLaunchKernelGGL(k_copy16, dim3(blocks_one_per_thread(bytes / 16)), dim3(256), 0, g_stream,
+                           (const copy_unit*)src->d_amps, (copy_unit*)dst->d_amps, bytes / 16);
+        LQ_LAUNCH_CHECK();
+    } else {
+        LQ_HIP_CHECK(hipMemcpyAsync(dst->d_amps, src->d_amps, bytes, hipMemcpyDeviceToDevice, g_stream));
+    }
     return LQ_OK;
 }
 
