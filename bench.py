@@ -232,56 +232,61 @@ def uneven_batch_bench(lq, reps=2):
     return res
 
 
-def warmstart_bench(lq, count=4, tile=None):
+def warmstart_bench(lq, live=2, same_method=2, tile=None):
     """Learned warm-start VQE at 20 qubits: XYZ rings with J ~ U[-1,1]^3,
     the L=8 RY/RZ + CZ-ring ansatz (P=320), network 3-256-256-320 trained on
-    the GPU (bf16 MFMA, f32 accumulate) on the committed converged solutions
-    of 4096 training couplings.  Time to reach within 1e-4 of the converged
-    energy: the plain GPU method (theta0, sequential parameter shift with
-    per-gate kernels, as VQE::run with ParameterShift) against the warm-
-    started batched method (network proposal with plain fallback, all 2P+1
-    shifted circuits batched).  Iteration counts come from the adjoint-
-    gradient trajectories, equal to the parameter-shift ones to rounding
-    (checked here on the first iterations); per-iteration wall times of both
-    methods are measured here."""
+    the GPU here (bf16 MFMA, f32 accumulate) on the committed converged
+    solutions of 4096 training couplings.
+
+    Target: within 1e-4 of E*, the lower final energy of two long runs
+    (Adam, adjoint gradient, 5000 iterations, from the plain and from the
+    warm start) of each of the first 32 held-out couplings, committed in
+    logosq_amd/data/warmstart_xyz20_L8_converged.json.  No run stops on
+    Adam's own tests within 5000 iterations, so E* is the best energy these
+    runs reach, not a certified minimum.
+
+    * Modelled over all 32 couplings: recorded iterations to the target x
+      measured per-iteration time of each method (plain: VQE::run with the
+      sequential parameter shift on the per-gate kernels; warm: network
+      proposal + VQE::run with the batched parameter shift).
+    * Measured end to end, as a check of that model: the warm-started
+      batched method on `live` couplings (the first whose recorded warm run
+      reached the target within 1000 iterations) -- network proposal, start
+      choice (f64 energies), VQE::run with the batched parameter shift (all
+      2P+1 circuits per pass launch), stopped at the first energy within the
+      target (lq_vqe_run_to_target) -- wall time.
+    * The warm start's own gain, measured end to end with the same gradient
+      method on both sides (adjoint): plain start against warm start to the
+      same target, first `same_method` couplings.
+    * The plain GPU method (VQE::run with sequential parameter shift on the
+      per-gate kernels, 2P+1 circuits one after another) takes ~1.8 s per
+      iteration and thousands of iterations: its time is modelled as the
+      recorded iterations x its measured per-iteration time, and labelled so."""
     import numpy as np
     from logosq_amd import warmstart
     path = os.path.join(ROOT, "logosq_amd", "data", "warmstart_xyz20_L8.npz")
     evp = os.path.join(ROOT, "logosq_amd", "data", "warmstart_xyz20_L8_eval.json")
-    if not os.path.exists(path):
-        return {"skipped": "training solutions not present (python -m logosq_amd.warmstart generate)"}
+    cvp = os.path.join(ROOT, "logosq_amd", "data", "warmstart_xyz20_L8_converged.json")
+    for pth in (path, evp, cvp):
+        if not os.path.exists(pth):
+            return {"skipped": "%s not present" % os.path.relpath(pth, ROOT)}
     J, X, E, it, solved, theta0, meta = warmstart.load_data(path)
-    if not solved.all():
-        return {"skipped": "training solutions incomplete: %d of %d" % (solved.sum(), len(J))}
-    ev = None
-    if os.path.exists(evp):
-        with open(evp) as f:
-            ev = json.load(f)
+    with open(evp) as f:
+        ev = json.load(f)
+    with open(cvp) as f:
+        cv = json.load(f)
     n, layers = meta["num_qubits"], meta["layers"]
     ans = lq.ry_rz_cz_ring_ansatz(n, layers)
-    lr_p = ev["lr_plain"] if ev else meta["learning_rate"]
-    lr_w = ev["lr_warm"] if ev else meta["learning_rate"]
-    cap, tol = (ev["max_iterations"], ev["tolerance"]) if ev and "max_iterations" in ev else (1500, 1e-10)
+    lr_p, lr_w = cv["lr_plain"], cv["lr_warm"]
+    cap = cv["max_iterations"]
     t0 = time.perf_counter()
-    # the training recipe the evaluation used (one stage, or the default schedule)
-    ws, hist = warmstart.WarmStart.train(J, X, theta0, epochs=ev.get("epochs") if ev else None,
-                                         learning_rate=ev.get("lr") if ev else None)
+    ws, hist = warmstart.WarmStart.train(J, X, theta0, epochs=cv.get("epochs"), learning_rate=cv.get("lr"))
     t_train = time.perf_counter() - t0
-    Jte = warmstart.couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][:count]
-    # the recorded evaluation ran the generic tile kernel at the default tile
-    # geometry, and so does its live re-check (equal iteration counts show
-    # the record reproduces; the runs are chaotic, so any other rounding
-    # moves them)
-    lq.jit.set_jit(lq.jit.OFF)
-    lq._lib.call("lq_set_fused_config", 0, 0, 0)
-    try:
-        rows = warmstart.evaluate(ans, ws, Jte, theta0, lr_p, lr_w, cap, tol)
-    finally:
-        lq.jit.set_jit(lq.jit.REPEAT)
-        if tile:
-            lq._lib.call("lq_set_fused_config", *tile)
-    # per-iteration wall time of each method at the first held-out coupling
-    obs = warmstart.hamiltonians(n, Jte[:1])[0]
+    rows = cv["rows"]
+    Jc = np.array([r["J"] for r in rows])
+    if tile:
+        lq._lib.call("lq_set_fused_config", *tile)
+    obs0 = warmstart.hamiltonians(n, Jc[:1])[0]
     seq, bat, adj = lq.ParameterShift(batched=False), lq.ParameterShift(batched=True), lq.AdjointGradient()
 
     def wall(fn, reps):
@@ -292,74 +297,83 @@ def warmstart_bench(lq, count=4, tile=None):
             fn()
             ts.append(time.perf_counter() - t)
         return min(ts)
-    t_seq = wall(lambda: seq.compute_gradient(ans, obs, theta0, energy=True), 1)
-    t_seqf = wall(lambda: sequential_fused_ps(lq, ans, obs, theta0), 1)
-    g_sf = sequential_fused_ps(lq, ans, obs, theta0)[1]
-    g_bat = np.array(bat.compute_gradient(ans, obs, theta0))
-    t_bat = wall(lambda: bat.compute_gradient(ans, obs, theta0, energy=True), 2)
-    lq.jit.set_jit(lq.jit.OFF)
-    t_bat_generic = wall(lambda: bat.compute_gradient(ans, obs, theta0, energy=True), 2)
-    lq.jit.set_jit(lq.jit.REPEAT)
-    t_adj = wall(lambda: adj.compute_gradient(ans, obs, theta0, energy=True), 3)
-    t_start = wall(lambda: warmstart.choose_starts(ws, ans, Jte[:1]), 2)
-    # the batched parameter-shift VQE follows the adjoint trajectory
-    start = warmstart.choose_starts(ws, ans, Jte[:1])[0][0]
-    r_ps = lq.VQE(ans, obs, bat, lq.Adam(lr_w, 3)).run(start)
-    r_adj = lq.VQE(ans, obs, adj, lq.Adam(lr_w, 3)).run(start)
-    traj = float(np.max(np.abs(np.array(r_ps.convergence_history) - r_adj.convergence_history)))
+    # per-iteration times (one gradient with its energy) at the plain start
+    t_seq = wall(lambda: seq.compute_gradient(ans, obs0, theta0, energy=True), 1)
+    t_seqf = wall(lambda: sequential_fused_ps(lq, ans, obs0, theta0), 1)
+    g_sf = sequential_fused_ps(lq, ans, obs0, theta0)[1]
+    g_bat = np.array(bat.compute_gradient(ans, obs0, theta0))
+    t_bat = wall(lambda: bat.compute_gradient(ans, obs0, theta0, energy=True), 2)
+    t_adj = wall(lambda: adj.compute_gradient(ans, obs0, theta0, energy=True), 3)
+    t_start = wall(lambda: warmstart.choose_starts(ws, ans, Jc[:1]), 2)
 
-    def times(rs, kp, kw, t_plain=t_seq, t_warm=t_bat):
-        tp = [(r[kp] or cap) * t_plain for r in rs]
-        tw = [t_start + (r[kw] or cap) * t_warm for r in rs]
-        return tp, tw
+    def to_target(j, row, method, lr, warm, max_it):
+        """Wall time from the couplings to the first energy within the
+        target: (seconds, iterations, reached)."""
+        obs = warmstart.hamiltonians(n, [j])[0]
+        t = time.perf_counter()
+        x0 = warmstart.choose_starts(ws, ans, [j])[0][0] if warm else theta0
+        opt = lq.Adam(lr, max_it).with_tolerance(cv["tolerance"])  # the record's stopping tests
+        r, reached = lq.VQE(ans, obs, method, opt).run_to_target(x0, row["e_star"] + 1e-4)
+        lq._lib.call("lq_synchronize")
+        return time.perf_counter() - t, len(r.convergence_history), reached
 
-    def summary(rs):
-        out = {}
-        # dE < 1e-4 against the plain method's converged f64 energy (primary),
-        # and against the lower of the two methods' converged energies
-        for name, kp, kw in (("vs_plain_converged_energy", "iters_plain_to_plain", "iters_warm_to_plain"),
-                             ("vs_best_converged_energy", "iters_plain", "iters_warm")):
-            tp, tw = times(rs, kp, kw)
-            out[name] = {"reached_plain": sum(r[kp] is not None for r in rs),
-                         "reached_warm": sum(r[kw] is not None for r in rs),
-                         "median_iterations_plain": float(np.median([r[kp] or cap for r in rs])),
-                         "median_iterations_warm": float(np.median([r[kw] or cap for r in rs])),
-                         "median_time_to_dE_s_plain": round(float(np.median(tp)), 2),
-                         "median_time_to_dE_s_warm_batched": round(float(np.median(tw)), 2),
-                         "speedup_median": round(float(np.median(tp) / np.median(tw)), 2),
-                         "speedup_total": round(float(np.sum(tp) / np.sum(tw)), 2)}
-            # the same iterations against the strongest one-circuit-at-a-time
-            # plain method (fused execution), and with the adjoint gradient
-            tpf, _ = times(rs, kp, kw, t_plain=t_seqf)
-            _, twa = times(rs, kp, kw, t_warm=t_adj)
-            out[name]["speedup_median_vs_plain_fused_sequential"] = round(float(np.median(tpf) / np.median(tw)), 2)
-            out[name]["speedup_median_warm_adjoint_vs_plain"] = round(float(np.median(tp) / np.median(twa)), 1)
-        out["warm_final_energy_lower"] = sum(r["e_warm_final"] < r["e_plain_final"] - 1e-4 for r in rs)
-        out["proposal_used"] = sum(r["used_proposal"] for r in rs)
-        return out
-
-    out = {"workload": "20q XYZ ring, L=8 RY/RZ+CZ (P=320), 4096 training solutions; %d held-out couplings run "
-                       "live, iteration cap %d" % (len(rows), cap),
+    # live sample: warm runs take ~50 ms per iteration and thousands of
+    # iterations for most couplings, so the end-to-end check of the model
+    # runs on the first couplings whose recorded warm run reached the target
+    # within 1000 iterations
+    pick = [i for i, r in enumerate(rows) if r["iters_warm"] is not None and r["iters_warm"] <= 1000][:live]
+    measured = []
+    for j, row in zip(Jc[pick], [rows[i] for i in pick]):
+        max_it = min(cap, 2 * (row["iters_warm"] or cap) + 200)
+        s_, k_, ok = to_target(j, row, bat, lr_w, True, max_it)
+        measured.append({"coupling": pick[len(measured)], "J": [round(v, 6) for v in j], "s": round(s_, 3),
+                         "iterations": k_, "reached": ok,
+                         "modelled_s": round(t_start + (row["iters_warm"] or cap) * t_bat, 2),
+                         "recorded_iterations_warm": row["iters_warm"],
+                         "plain_modelled_s": round((row["iters_plain"] or cap) * t_seq, 1),
+                         "recorded_iterations_plain": row["iters_plain"]})
+        print("bench: warm batched VQE to target %.2f s (%d iterations)" % (s_, k_), file=sys.stderr, flush=True)
+    own = []
+    for j, row in zip(Jc[:same_method], rows[:same_method]):
+        sp, kp, okp = to_target(j, row, adj, lr_p, False, cap)
+        sw, kw, okw = to_target(j, row, adj, lr_w, True, cap)
+        own.append({"plain_s": round(sp, 3), "plain_iterations": kp, "plain_reached": okp,
+                    "warm_s": round(sw, 3), "warm_iterations": kw, "warm_reached": okw})
+    med = lambda v: float(np.median(v))
+    ip = [r["iters_plain"] or cap for r in rows]
+    iw = [r["iters_warm"] or cap for r in rows]
+    tp_model = [k * t_seq for k in ip]
+    tw_model = [t_start + k * t_bat for k in iw]
+    out = {"workload": "20q XYZ ring, L=8 RY/RZ+CZ (P=320), 4096 training solutions; E* from runs to convergence "
+                       "(%d held-out couplings, both starts, up to %d iterations)" % (len(rows), cap),
            "train_s": round(t_train, 2), "train_loss_first_last": [float(hist[0]), float(hist[-1])],
            "lr_plain": lr_p, "lr_warm": lr_w,
+           "learning_rates_from": "validation grid (0.02, 0.05, 0.1) per method, 16 couplings of another stream",
            "s_per_iteration": {"plain_sequential_ps": round(t_seq, 4),
                                "plain_sequential_ps_fused_execution": round(t_seqf, 4),
-                               "batched_ps": round(t_bat, 4),
-                               "batched_ps_generic_tile_kernel": round(t_bat_generic, 4),
-                               "adjoint": round(t_adj, 5), "warm_start_proposal": round(t_start, 4)},
-           "trajectory_batched_ps_vs_adjoint_max_abs": traj,
+                               "batched_ps": round(t_bat, 4), "adjoint": round(t_adj, 5),
+                               "warm_start_proposal": round(t_start, 4)},
            "gradient_fused_sequential_vs_batched_max_abs": float(np.max(np.abs(g_sf - g_bat))),
-           "live": summary(rows)}
-    if ev:
-        # the live couplings are the first held-out ones: same iterations
-        rec = {tuple(r["J"]): r for r in ev["rows"]}
-        same = [rec[tuple(r["J"])]["iters_warm_to_plain"] == r["iters_warm_to_plain"] and
-                rec[tuple(r["J"])]["iters_plain_to_plain"] == r["iters_plain_to_plain"]
-                for r in rows if tuple(r["J"]) in rec]
-        out["live_iterations_equal_recorded"] = "%d of %d" % (sum(same), len(same))
-        out["held_out_%d" % len(ev["rows"])] = summary(ev["rows"])
-        out["held_out_%d" % len(ev["rows"])]["source"] = ("logosq_amd/data/warmstart_xyz20_L8_eval.json (iterations), "
-                                                        "per-iteration times measured here")
+           "measured_warm_batched_end_to_end": measured,
+           "converged_record": {
+               "couplings": len(rows),
+               "runs_stopped_before_cap": {"plain": sum(r["stopped_plain"] for r in rows),
+                                           "warm": sum(r["stopped_warm"] for r in rows)},
+               "reached": {"plain": sum(r["iters_plain"] is not None for r in rows),
+                           "warm": sum(r["iters_warm"] is not None for r in rows)},
+               "median_iterations": {"plain": med(ip), "warm": med(iw)},
+               "warm_start_own_gain_iterations_median": round(med(ip) / med(iw), 2),
+               "median_time_to_dE_s": {"plain_sequential_ps_modelled": round(med(tp_model), 1),
+                                       "warm_batched_modelled": round(med(tw_model), 2)},
+               "speedup_median_modelled": round(med(tp_model) / med(tw_model), 1),
+               "speedup_median_modelled_vs_plain_fused_sequential":
+                   round(med([k * t_seqf for k in ip]) / med(tw_model), 1),
+               "note": "plain times modelled: recorded iterations x measured per-iteration time; unreached targets "
+                       "count as the cap"},
+           "warm_start_own_gain_adjoint_measured": {
+               "couplings": len(own), "runs": own,
+               "median_s": {"plain": med([o["plain_s"] for o in own]), "warm": med([o["warm_s"] for o in own])},
+               "speedup_median": round(med([o["plain_s"] for o in own]) / med([o["warm_s"] for o in own]), 2)}}
     return out
 
 

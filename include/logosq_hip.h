@@ -403,6 +403,17 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
                const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
                int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
                double* params, double* history, lq_state* final_state, lq_opt_result* result);
+/* VQE::run with Adam (grad_mode LQ_GRAD_*) that also stops as soon as a
+ * recorded energy is at or below target_energy (no reference counterpart:
+ * time-to-target measurements).  Until then every iteration is exactly
+ * VQE::run's; *reached (nullable) is 1 when the target stopped the run.
+ * The history holds the energies up to and including the first one at or
+ * below the target. */
+int lq_vqe_run_to_target(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+                         const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+                         int grad_mode, double learning_rate, int max_iterations, double tolerance,
+                         double target_energy, double* params, double* history, lq_opt_result* result,
+                         int32_t* reached);
 /* `count` VQE::run calls with Adam and the adjoint gradient, advanced in
  * lock step (no reference counterpart): instance i starts from
  * params[i*num_params ..] and minimises the Hamiltonian with coefficients

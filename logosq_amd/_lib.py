@@ -72,6 +72,8 @@ SIGNATURES = {
                             P_double],
     "lq_adjoint_gradient_batch": [c_int, c_int, P_gate, c_int, c_int, P_double, c_int, P_u8, P_double, c_int, c_int,
                                   P_double, P_double],
+    "lq_vqe_run_to_target": [c_int, c_int, P_gate, c_int, c_int, P_u8, P_double, c_int, c_int, c_double, c_int,
+                             c_double, c_int, c_double, c_double, P_double, P_double, ctypes.c_void_p, P_int],
     "lq_vqe_adam_batch": [c_int, c_int, P_gate, c_int, c_int, P_u8, P_double, c_int, c_int, c_int, c_double, c_int,
                           c_double, P_double, P_double, P_double, P_int, P_int],
     "lq_set_batch_graphs": [c_int],

@@ -60,16 +60,23 @@ int gd_minimize(double lr, int max_it, double tol, const Cb& cb, double* x, int 
     return LQ_OK;
 }
 
+// target: stop as soon as a recorded value is at or below it (-inf: never,
+// Adam::minimize itself); *reached (nullable) tells whether it was.
 int adam_minimize(double lr, double b1, double b2, double eps, int max_it, double tol, const Cb& cb, double* x,
-                  int dim, double* history, lq_opt_result* res) {
+                  int dim, double* history, lq_opt_result* res, double target = -INFINITY, bool* reached = nullptr) {
     std::vector<double> m(dim, 0.0), v(dim, 0.0), grad(dim > 0 ? dim : 1);
     int hist = 0;
     bool converged = false, have_prev = false;
     double prev = 0.0;
+    if (reached) *reached = false;
     for (int it = 1; it <= max_it; ++it) {
         double value = cb.f(x, dim, cb.user);
         if (history) history[hist] = value;
         ++hist;
+        if (value <= target) {
+            if (reached) *reached = true;
+            break;
+        }
         if (have_prev && std::fabs(prev - value) < tol) {
             converged = true;
             break;
@@ -207,11 +214,11 @@ int lq_adam_minimize(double learning_rate, double beta1, double beta2, double ep
                          result);
 }
 
-int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
-               const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
-               int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
-               double* params, double* history, lq_state* final_state, lq_opt_result* result) {
-    LQ_API_GUARD;
+static int vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+                   const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+                   int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
+                   double* params, double* history, lq_state* final_state, lq_opt_result* result, double target,
+                   bool* reached) {
     clear_error();
     if (!result || num_params < 0 || (num_params > 0 && !params)) {
         set_error("invalid VQE arguments");
@@ -242,7 +249,7 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
     Cb cb{vqe_f, vqe_g, &c};
     if (optimizer == 1)
         rc = adam_minimize(learning_rate, 0.9, 0.999, 1e-8, max_iterations, tolerance, cb, params, num_params, history,
-                           result);
+                           result, target, reached);
     else
         rc = gd_minimize(learning_rate, max_iterations, tolerance, cb, params, num_params, history, result);
     if (!rc) rc = c.rc;
@@ -256,6 +263,30 @@ int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, in
         }
     }
     lq_state_destroy(c.st);
+    return rc;
+}
+
+int lq_vqe_run(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+               const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+               int grad_mode, int optimizer, double learning_rate, int max_iterations, double tolerance,
+               double* params, double* history, lq_state* final_state, lq_opt_result* result) {
+    LQ_API_GUARD;
+    return vqe_run(num_qubits, dtype, tmpl, num_gates, num_params, paulis, coeffs, num_terms, convention, shift,
+                   grad_mode, optimizer, learning_rate, max_iterations, tolerance, params, history, final_state,
+                   result, -INFINITY, nullptr);
+}
+
+int lq_vqe_run_to_target(int num_qubits, int dtype, const lq_gate* tmpl, int num_gates, int num_params,
+                         const uint8_t* paulis, const double* coeffs, int num_terms, int convention, double shift,
+                         int grad_mode, double learning_rate, int max_iterations, double tolerance,
+                         double target_energy, double* params, double* history, lq_opt_result* result,
+                         int32_t* reached) {
+    LQ_API_GUARD;
+    bool r = false;
+    int rc = vqe_run(num_qubits, dtype, tmpl, num_gates, num_params, paulis, coeffs, num_terms, convention, shift,
+                     grad_mode, 1, learning_rate, max_iterations, tolerance, params, history, nullptr, result,
+                     target_energy, &r);
+    if (reached) *reached = r ? 1 : 0;
     return rc;
 }
 

@@ -599,6 +599,30 @@ class VQE:
              final.handle, ctypes.byref(res))
         return VQEResult(res.optimal_value, x.tolist(), hist[:res.num_iterations].tolist(), final)
 
+    def run_to_target(self, initial_parameters, target_energy):
+        """VQE::run with Adam that also stops at the first energy at or below
+        target_energy (lq_vqe_run_to_target; no reference counterpart).
+        Returns (VQEResult without final state, reached)."""
+        arr, ng = _template_args(self.ansatz)
+        codes, coeffs = _obs_args(self.hamiltonian)
+        x = np.array(initial_parameters, dtype=np.float64, copy=True)
+        opt = self.optimizer
+        if not isinstance(opt, Adam):
+            raise ValueError("run_to_target drives Adam")
+        hist = np.zeros(max(1, opt.max_iterations))
+        res = LqOptResult()
+        gm = self.gradient_method
+        mode = getattr(gm, "mode", abi.GRAD_BATCHED if getattr(gm, "batched", False) else abi.GRAD_SEQUENTIAL)
+        dtype = getattr(gm, "dtype", C128)
+        reached = ctypes.c_int()
+        call("lq_vqe_run_to_target", self.ansatz.num_qubits(), dtype, arr, ng, len(x),
+             codes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+             coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(coeffs), self.hamiltonian.convention,
+             gm.shift, mode, opt.learning_rate, opt.max_iterations, opt.tolerance, float(target_energy),
+             x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+             ctypes.byref(res), ctypes.byref(reached))
+        return VQEResult(res.optimal_value, x.tolist(), hist[:res.num_iterations].tolist(), None), bool(reached.value)
+
     def run_random(self, seed=None):
         rng = np.random.default_rng(seed)
         return self.run(rng.uniform(0.0, 2.0 * math.pi, self.ansatz.num_parameters()))

@@ -402,6 +402,50 @@ def cmd_evaluate(a):
         np.median([v for v in iw if v is not None]) if any(v is not None for v in iw) else None), flush=True)
 
 
+def cmd_converge(a):
+    """E* by runs to convergence: for the first `count` held-out couplings,
+    VQE::run with Adam (adjoint gradient) from the plain and the warm starts
+    at the evaluation's learning rates, up to `max_iterations` iterations at
+    `tolerance`; E* is the lower of the two final energies.  Records per
+    coupling E*, both final energies and iteration counts to E* + 1e-4, and
+    whether each run stopped by Adam's own tests before the cap."""
+    from .optimization import Adam, ry_rz_cz_ring_ansatz
+    J, X, E, it, solved, theta0, meta = load_data(a.data)
+    with open(a.eval) as f:
+        ev = json.load(f)
+    ans = ry_rz_cz_ring_ansatz(meta["num_qubits"], meta["layers"])
+    ws, hist = WarmStart.train(J, X, theta0, epochs=ev.get("epochs"), learning_rate=ev.get("lr"))
+    print("trained: loss %.4g -> %.4g" % (hist[0], hist[-1]), flush=True)
+    Jte = couplings(meta["train"] + meta["held_out"], meta["seed"])[meta["train"]:][:a.count]
+    starts, used, ep, e0 = choose_starts(ws, ans, Jte)
+    lr_p, lr_w = ev["lr_plain"], ev["lr_warm"]
+    log = lambda m: print(m, flush=True)
+    _, e_p, h_p, c_p = solve(ans, Jte, np.tile(theta0, (len(Jte), 1)),
+                             Adam(lr_p, a.max_iterations).with_tolerance(a.tolerance), batch=a.batch,
+                             log=lambda m: log("plain: " + m))
+    _, e_w, h_w, c_w = solve(ans, Jte, starts, Adam(lr_w, a.max_iterations).with_tolerance(a.tolerance),
+                             batch=a.batch, log=lambda m: log("warm: " + m))
+    rows = []
+    for i in range(len(Jte)):
+        e_star = float(min(e_p[i], e_w[i]))
+        rows.append({"J": [float(v) for v in Jte[i]], "e_star": e_star, "e_plain_final": float(e_p[i]),
+                     "e_warm_final": float(e_w[i]), "used_proposal": bool(used[i]),
+                     "iters_plain": iterations_to(h_p[i], e_star + 1e-4),
+                     "iters_warm": iterations_to(h_w[i], e_star + 1e-4),
+                     "len_plain": len(h_p[i]), "len_warm": len(h_w[i]),
+                     "stopped_plain": bool(c_p[i]), "stopped_warm": bool(c_w[i])})
+    with open(a.out, "w") as f:
+        json.dump({"meta": meta, "lr_plain": lr_p, "lr_warm": lr_w, "epochs": ev.get("epochs"), "lr": ev.get("lr"),
+                   "max_iterations": a.max_iterations, "tolerance": a.tolerance, "count": len(rows), "rows": rows}, f)
+    ip = [r["iters_plain"] for r in rows]
+    iw = [r["iters_warm"] for r in rows]
+    print("converged %d: plain reached %d (median %s), warm reached %d (median %s); stopped before the cap: "
+          "plain %d, warm %d" % (len(rows), sum(v is not None for v in ip),
+                                 np.median([v or a.max_iterations for v in ip]), sum(v is not None for v in iw),
+                                 np.median([v or a.max_iterations for v in iw]), sum(r["stopped_plain"] for r in rows),
+                                 sum(r["stopped_warm"] for r in rows)), flush=True)
+
+
 def cmd_merge(a):
     """Join evaluation parts (disjoint --offset ranges of the held-out
     couplings, same training recipe and learning rates) into one file."""
@@ -474,6 +518,14 @@ def main(argv=None):
     e.add_argument("--lr-plain", type=float, default=None)
     e.add_argument("--lr-warm", type=float, default=None)
     e.add_argument("--lr-grid", default="0.02,0.05,0.1")
+    c = sub.add_parser("converge", help="E* of held-out couplings from runs to convergence, both starts")
+    c.add_argument("--data", required=True)
+    c.add_argument("--eval", required=True, help="the evaluation record (learning rates, training recipe)")
+    c.add_argument("--out", required=True)
+    c.add_argument("--count", type=int, default=32)
+    c.add_argument("--max-iterations", type=int, default=5000)
+    c.add_argument("--tolerance", type=float, default=1e-12)
+    c.add_argument("--batch", type=int, default=16)
     m = sub.add_parser("merge", help="join evaluation parts into one file")
     m.add_argument("--out", required=True)
     m.add_argument("parts", nargs="+")
@@ -489,7 +541,7 @@ def main(argv=None):
     t.add_argument("--jit", choices=["off", "repeat", "always"], default="repeat",
                    help="run-time specialised tile passes (lq_set_jit)")
     a = ap.parse_args(argv)
-    {"generate": cmd_generate, "evaluate": cmd_evaluate, "merge": cmd_merge,
+    {"generate": cmd_generate, "evaluate": cmd_evaluate, "converge": cmd_converge, "merge": cmd_merge,
      "time-gradient": cmd_time_gradient}[a.cmd](a)
 
 

@@ -12,7 +12,8 @@ import numpy as np
 from logosq_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "liblq_oracle.so")
+# LQ_ORACLE_LIB selects another build (the sanitized one: make -C oracle sanitize)
+LIB = os.environ.get("LQ_ORACLE_LIB", os.path.join(HERE, "_build", "liblq_oracle.so"))
 
 _lib = None
 
@@ -37,7 +38,7 @@ GRD = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctyp
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if not os.path.exists(LIB) and "LQ_ORACLE_LIB" not in os.environ:
             build()
         L = ctypes.CDLL(LIB)
         d, i, u64, vp = ctypes.c_double, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p

@@ -63,6 +63,9 @@ def flip_groups(n, codes):
 
 
 def run_layouts(tmp_path, plan, psi, dtype):
+    # the compiler and the emulator carry their own sanitizer runtime: no
+    # preloaded one (a sanitized library run preloads it into this process)
+    env = {k: v for k, v in os.environ.items() if k != "LD_PRELOAD"}
     state = tmp_path / "state.bin"
     np.ascontiguousarray(psi, dtype=np.complex128).view(np.float64).tofile(state)
     total = 0.0
@@ -75,10 +78,10 @@ def run_layouts(tmp_path, plan, psi, dtype):
         cmd = [CLANG, "-std=c++20", "-O1", "-g", "-fsanitize=address", "-pthread", "-I", os.path.join(HERE, "emul"),
                '-DLQ_KERNEL_FILE="%s"' % src, "-DLQ_V=%s" % ("double2" if dtype == lq.C128 else "float2"),
                os.path.join(HERE, "emul", "run_layout.cpp"), "-o", str(exe)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env)
         assert r.returncode == 0, r.stderr[-3000:]
         r = subprocess.run([str(exe), str(plan["n"]), str(dtype), str(plan["blocks"]), str(lay["threads"]), str(state),
-                            str(coef)], capture_output=True, text=True, timeout=300)
+                            str(coef)], capture_output=True, text=True, timeout=300, env=env)
         assert r.returncode == 0, r.stderr[-3000:]
         total += float(r.stdout)
     return total

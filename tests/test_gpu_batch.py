@@ -252,3 +252,26 @@ def algorithms_passes(n, gates):
                              ctypes.byref(npass)) == 0
     return npass.value
 
+
+
+def test_vqe_run_to_target_follows_vqe_run(hip):
+    """lq_vqe_run_to_target: the same iterations as VQE::run (Adam,
+    src/optimization/optimizer.rs:139-223) up to the first energy at or below
+    the target, where it stops; with an unreachable target it is VQE::run."""
+    say("vqe run to target")
+    n = 6
+    ans = lq.ry_rz_cz_ring_ansatz(n, 2)
+    obs = lq.heisenberg_ring(n, 0.4, -0.8, 0.3)
+    x0 = np.linspace(0.2, 2.2, ans.num_parameters())
+    for grad in (lq.AdjointGradient(), lq.ParameterShift(batched=True), lq.ParameterShift()):
+        full = lq.VQE(ans, obs, grad, lq.Adam(0.05, 40)).run(x0)
+        h = np.array(full.convergence_history)
+        assert len(h) == 40
+        target = float(h[9])
+        first = int(np.flatnonzero(h <= target)[0])
+        r, reached = lq.VQE(ans, obs, grad, lq.Adam(0.05, 40)).run_to_target(x0, target)
+        assert reached and len(r.convergence_history) == first + 1
+        assert np.array_equal(np.array(r.convergence_history), h[:first + 1])
+        r2, reached2 = lq.VQE(ans, obs, grad, lq.Adam(0.05, 40)).run_to_target(x0, -1e9)
+        assert not reached2 and np.array_equal(np.array(r2.convergence_history), h)
+        assert np.max(np.abs(np.array(r2.optimal_parameters) - np.array(full.optimal_parameters))) == 0.0
