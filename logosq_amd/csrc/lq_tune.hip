@@ -131,9 +131,20 @@ extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t repor
                 if (!rc) rc = time_ms(e0, e1, &ms);
             }
             if (rc) break;
-            char buf[128];
-            snprintf(buf, sizeof(buf), "%s{\"tile_bits\": %d, \"low_bits\": %d, \"ms\": %.4f}", first ? "" : ", ", tb,
-                     lows[li], ms);
+            // passes of this geometry's plan and their rate (each reads and writes the state once)
+            std::vector<Prim> prims;
+            int passes = 0;
+            if (gates_to_prims(n, circ.data(), (int)circ.size(), prims) == LQ_OK) {
+                Plan plan;
+                plan_passes(n, prims, tb, lows[li], plan);
+                passes = plan.passes;
+            }
+            const double gbs = ms > 0.f ? passes * 2.0 * (dtype == LQ_C128 ? 16.0 : 8.0) * (double)(1ull << n) / ms / 1e6
+                                        : 0.0;
+            char buf[192];
+            snprintf(buf, sizeof(buf),
+                     "%s{\"tile_bits\": %d, \"low_bits\": %d, \"ms\": %.4f, \"passes\": %d, \"pass_GBps\": %.1f}",
+                     first ? "" : ", ", tb, lows[li], ms, passes, gbs);
             rep += buf;
             first = false;
             if (ms < bestf) {
