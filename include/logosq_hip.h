@@ -214,6 +214,21 @@ int lq_set_fused_config(int tile_bits_c128, int tile_bits_c64, int low_bits);
  * wavefront (1, 2, 4 or 8; default 1) and nontemporal global access on/off
  * (default on). */
 int lq_set_gate_kernel_config(int unroll, int nontemporal);
+/* Work-unit order of the per-gate kernels (no counterpart in the reference):
+ * a mask of kernel classes whose work units run in a 32 x 32-transposed
+ * order inside each block of 1024 units, so that 32 consecutive wavefronts
+ * take units 32 apart (launches of at least 1024 units, i.e. 2^16 free
+ * amplitudes; smaller ones keep the plain order).  Default LQ_UNITS_ALL; -1
+ * restores the default.  Each unit is visited once in either order, so
+ * the result is the same bit for bit; only the rate changes. */
+#define LQ_UNITS_PAIR_HI 1  /* pair update, target above the lane bits      */
+#define LQ_UNITS_PAIR_LO 2  /* pair update, target among the lane bits      */
+#define LQ_UNITS_DIAG 4     /* diagonal gates                               */
+#define LQ_UNITS_SWAP 8     /* SWAP                                         */
+#define LQ_UNITS_QUAD 16    /* 4x4 blocks, both bits above the lane bits    */
+#define LQ_UNITS_QUAD_LO 32 /* 4x4 blocks with a bit among the lane bits    */
+#define LQ_UNITS_ALL 63
+int lq_set_gate_unit_order(int classes);
 /* Autotuner (no counterpart in the reference): times the per-gate kernel
  * variants and the fused tile geometries on a num_qubits state of dtype
  * (allocated for the call), keeps the fastest of each, and writes a JSON

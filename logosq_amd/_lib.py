@@ -62,6 +62,7 @@ SIGNATURES = {
     "lq_execute": [c_void_p, P_gate, c_int, c_int],
     "lq_set_fused_config": [c_int, c_int, c_int],
     "lq_set_gate_kernel_config": [c_int, c_int],
+    "lq_set_gate_unit_order": [c_int],
     "lq_autotune": [c_int, c_int, ctypes.c_char_p, ctypes.c_size_t],
     "lq_plan_fused": [c_int, c_int, P_gate, c_int, c_int, c_int, P_int, P_int, P_int],
     "lq_expect_pauli_sum": [c_void_p, P_u8, P_double, c_int, c_int, P_double],

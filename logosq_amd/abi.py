@@ -32,6 +32,7 @@ PAULI_REFERENCE, PAULI_PHYSICAL = 0, 1
 EXPECT_TILES_OFF, EXPECT_TILES_AUTO, EXPECT_TILES_ON = 0, 1, 2
 
 EXEC_PER_GATE, EXEC_FUSED = 0, 1
+UNITS_PAIR_HI, UNITS_PAIR_LO, UNITS_DIAG, UNITS_SWAP, UNITS_QUAD, UNITS_QUAD_LO, UNITS_ALL = 1, 2, 4, 8, 16, 32, 63
 GRAD_SEQUENTIAL, GRAD_BATCHED, GRAD_ADJOINT = 0, 1, 2
 
 ARITY = {X: 1, Y: 1, Z: 1, H: 1, S: 1, T: 1, RX: 1, RY: 1, RZ: 1, PHASE: 1, U3: 1, MATRIX1: 1,

@@ -145,6 +145,8 @@ int launch_dense(lq_state* st, const double* d_matrix, uint64_t dim);
 
 // Per-gate kernel variant (work units per wave, nontemporal access).
 int set_gate_kernel_config(int unroll, int nontemporal);
+int set_gate_unit_order(int classes);
+int gate_unit_order();
 
 // Reductions (lq_state.hip).
 int reduce_norm_sqr(const lq_state* st, double* d_out);    // device scalar

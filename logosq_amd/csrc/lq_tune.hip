@@ -41,6 +41,12 @@ extern "C" int lq_set_gate_kernel_config(int unroll, int nontemporal) {
     return set_gate_kernel_config(unroll, nontemporal);
 }
 
+extern "C" int lq_set_gate_unit_order(int classes) {
+    LQ_API_GUARD;
+    clear_error();
+    return set_gate_unit_order(classes);
+}
+
 extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t report_len) {
     LQ_API_GUARD;
     clear_error();
@@ -93,8 +99,36 @@ extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t repor
     if (!rc) set_gate_kernel_config(best_u, best_nt);
     {
         char buf[128];
-        snprintf(buf, sizeof(buf), "], \"gate_choice\": {\"unroll\": %d, \"nontemporal\": %d}, \"tile_variants\": [",
+        snprintf(buf, sizeof(buf), "], \"gate_choice\": {\"unroll\": %d, \"nontemporal\": %d}, \"unit_orders\": [",
                  best_u, best_nt);
+        rep += buf;
+    }
+    // ---- work-unit order of the per-gate kernels: plain or transposed for all classes
+    {
+        const int orders[2] = {0, LQ_UNITS_ALL};
+        float besto = 1e30f;
+        int best_order = LQ_UNITS_ALL;
+        for (int oi = 0; oi < 2 && !rc; ++oi) {
+            set_gate_unit_order(orders[oi]);
+            float ms = 0.f;
+            for (int rep_i = 0; rep_i < 2 && !rc; ++rep_i) {
+                (void)hipEventRecord(e0, stream());
+                for (auto& g : probe) rc = rc ? rc : apply_gate_desc(st, &g);
+                (void)hipEventRecord(e1, stream());
+                if (!rc) rc = time_ms(e0, e1, &ms);
+            }
+            if (rc) break;
+            char buf[96];
+            snprintf(buf, sizeof(buf), "%s{\"classes\": %d, \"ms\": %.4f}", oi ? ", " : "", orders[oi], ms);
+            rep += buf;
+            if (ms < besto) {
+                besto = ms;
+                best_order = orders[oi];
+            }
+        }
+        if (!rc) set_gate_unit_order(best_order);
+        char buf[128];
+        snprintf(buf, sizeof(buf), "], \"unit_order_choice\": %d, \"tile_variants\": [", best_order);
         rep += buf;
     }
     // ---- fused tiles: a depth-4 brickwork of RX/RY/RZ + CNOT layers

@@ -76,6 +76,13 @@ pub const LQ_EXPECT_TILES_ON: c_int = 2;
 pub const LQ_GRAD_SEQUENTIAL: c_int = 0;
 pub const LQ_GRAD_BATCHED: c_int = 1;
 pub const LQ_GRAD_ADJOINT: c_int = 2;
+pub const LQ_UNITS_PAIR_HI: c_int = 1;
+pub const LQ_UNITS_PAIR_LO: c_int = 2;
+pub const LQ_UNITS_DIAG: c_int = 4;
+pub const LQ_UNITS_SWAP: c_int = 8;
+pub const LQ_UNITS_QUAD: c_int = 16;
+pub const LQ_UNITS_QUAD_LO: c_int = 32;
+pub const LQ_UNITS_ALL: c_int = 63;
 pub const LQ_JIT_OFF: c_int = 0;
 pub const LQ_JIT_REPEAT: c_int = 1;
 pub const LQ_JIT_ALWAYS: c_int = 2;
@@ -169,6 +176,7 @@ extern "C" {
     pub fn lq_execute(st: *mut LqState, gates: *const LqGate, count: c_int, mode: c_int) -> c_int;
     pub fn lq_set_fused_config(tile_bits_c128: c_int, tile_bits_c64: c_int, low_bits: c_int) -> c_int;
     pub fn lq_set_gate_kernel_config(unroll: c_int, nontemporal: c_int) -> c_int;
+    pub fn lq_set_gate_unit_order(classes: c_int) -> c_int;
     pub fn lq_autotune(num_qubits: c_int, dtype: c_int, report: *mut c_char, report_len: usize) -> c_int;
     pub fn lq_plan_fused(num_qubits: c_int, dtype: c_int, gates: *const LqGate, count: c_int, tile_bits: c_int, low_bits: c_int, gate_order: *mut c_int, pass_index: *mut c_int, num_passes: *mut c_int) -> c_int;
 

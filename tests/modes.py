@@ -15,3 +15,4 @@ def restore_defaults():
     lq.load()
     jit.set_jit(jit.ALWAYS if forced() else jit.REPEAT)
     lq._lib.call("lq_set_expect_tiles", abi.EXPECT_TILES_ON if forced() else abi.EXPECT_TILES_AUTO)
+    lq._lib.call("lq_set_gate_unit_order", -1)

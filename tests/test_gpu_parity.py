@@ -68,6 +68,74 @@ def test_gate_kernel_variants(hip, unroll, nt):
         lq._lib.call("lq_set_gate_kernel_config", 1, 1)
 
 
+def order_gates(n, rng):
+    """Every gate kind at low, middle and top placements (pairs both ways
+    round), so each per-gate kernel class runs at least once."""
+    from tests.test_oracle import ONE_Q, ONE_Q_ROT, TWO_Q, TWO_Q_ROT
+    out = []
+    for q in sorted({0, 1, 3, 6, n // 2, n - 2, n - 1}):
+        out += [G(k, [q]) for k in ONE_Q]
+        out += [G(k, [q], [rng.uniform(0, 2 * math.pi)]) for k in ONE_Q_ROT]
+        out.append(G(abi.U3, [q], rng.uniform(0, 2 * math.pi, 3)))
+    for a, b in [(0, 1), (1, 0), (0, n - 1), (n - 1, 0), (2, 9), (9, 2), (n // 2, n // 2 + 1), (n - 2, n - 1),
+                 (n - 1, n - 3), (4, 12)]:
+        out += [G(k, [a, b]) for k in TWO_Q]
+        out += [G(k, [a, b], [rng.uniform(0, 2 * math.pi)]) for k in TWO_Q_ROT]
+    out += [G(abi.TOFFOLI, t) for t in ([0, 1, 2], [n - 1, 0, n // 2], [3, n - 2, 7])]
+    return out
+
+
+@pytest.mark.parametrize("n,dtype", [(17, lq.C128), (18, lq.C64), (20, lq.C128)])
+def test_gate_unit_orders(hip, n, dtype):
+    """The transposed work-unit order (lq_set_gate_unit_order) visits every
+    unit once: with it on for every kernel class the state after a sequence
+    of every gate kind equals the plain order's bit for bit, and the oracle's
+    to the usual tolerance.  States of 2^17 amplitudes and more, so that the
+    launches have the 1024 units the order needs."""
+    say("unit orders n=%d dtype=%d" % (n, dtype))
+    rng = np.random.default_rng(4100 + n)
+    gates = order_gates(n, rng)
+    psi = bf.random_state(n, rng)
+    try:
+        got = {}
+        for classes in (0, abi.UNITS_ALL):
+            lq._lib.call("lq_set_gate_unit_order", classes)
+            got[classes] = gpu_apply(psi, n, gates, dtype)
+        assert np.array_equal(got[0], got[abi.UNITS_ALL])
+        want = oracle.execute(psi, n, gates)
+        tol = AMP_TOL if dtype == lq.C128 else 1e-5
+        assert np.max(np.abs(got[abi.UNITS_ALL] - want)) <= tol
+    finally:
+        lq._lib.call("lq_set_gate_unit_order", -1)
+
+
+def test_autotune_report(hip):
+    """lq_autotune at 16 qubits: the JSON report lists every gate variant,
+    both unit orders and the tile geometries with a choice for each, and
+    the configuration it leaves gives the oracle's results."""
+    import ctypes
+    import json
+    say("autotune 16 qubits")
+    buf = ctypes.create_string_buffer(16384)
+    try:
+        lq._lib.call("lq_autotune", 16, lq.C128, buf, len(buf))
+        rep = json.loads(buf.value.decode())
+        assert len(rep["gate_variants"]) == 8 and rep["gate_choice"]["unroll"] in (1, 2, 4, 8)
+        assert [o["classes"] for o in rep["unit_orders"]] == [0, abi.UNITS_ALL]
+        assert rep["unit_order_choice"] in (0, abi.UNITS_ALL)
+        assert rep["tile_variants"] and all(v["ms"] > 0 for v in rep["tile_variants"])
+        rng = np.random.default_rng(4200)
+        gates = order_gates(16, rng)
+        psi = bf.random_state(16, rng)
+        want = oracle.execute(psi, 16, gates)
+        assert np.max(np.abs(gpu_apply(psi, 16, gates) - want)) <= AMP_TOL
+        assert np.max(np.abs(gpu_apply(psi, 16, gates, mode=abi.EXEC_FUSED) - want)) <= AMP_TOL
+    finally:
+        lq._lib.call("lq_set_gate_kernel_config", 1, 1)
+        lq._lib.call("lq_set_gate_unit_order", -1)
+        lq._lib.call("lq_set_fused_config", 0, 0, 0)
+
+
 @pytest.mark.parametrize("n", [7, 12])
 def test_gates_complex64(hip, n):
     say("c64 gates n=%d" % n)
