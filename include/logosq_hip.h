@@ -300,8 +300,9 @@ int lq_expect_pauli_sum_noalloc(const lq_state* st, const uint8_t* paulis, const
 #define LQ_EXPECT_TILES_ON 2
 int lq_set_expect_tiles(int mode);
 /* The reads of the state an expectation of this sum makes in the current
- * modes: tile layouts, flip groups read on their own, diagonal groups read
- * on their own.  With run-time specialisation on (lq_set_jit) and tiles
+ * modes: tile layouts; reads for the flip groups outside the tiles (orbit
+ * items: up to four linearly independent flip masks per read); diagonal
+ * groups read on their own.  With run-time specialisation on (lq_set_jit) and tiles
  * wanted for the size, the layouts are those of the specialised kernels
  * (128 KiB tiles, the diagonal group inside the first layout), used once
  * they compile. */

@@ -74,19 +74,6 @@ __global__ void k_fill_zero_states(V* __restrict__ psi, uint64_t dim, uint64_t t
     }
 }
 
-// One work item: a flip-mask group of one instance's Pauli sum.
-struct ExpItem {
-    uint64_t state_off;  // amplitude offset of the instance's state
-    uint64_t x;          // flip mask; 0 = diagonal group
-    uint64_t work;       // amplitudes (diagonal) or pairs (flip) to visit
-    int32_t h;           // top bit of x
-    int32_t term_first;
-    int32_t term_count;
-    int32_t block_first; // first block of this item in the launch
-    int32_t nblocks;
-    int32_t inst;
-};
-
 // One op whose matrix differs from instance 0's in instance `inst`.
 struct OpOverride {
     int32_t inst, op;
@@ -105,119 +92,6 @@ __device__ __forceinline__ double bsum(double v, double* sh) {
         __syncthreads();
     }
     return sh[0];
-}
-
-template <typename V>
-__global__ void __launch_bounds__(kEThreads) k_expect_items(const V* __restrict__ psi,
-                                                            const ExpItem* __restrict__ items, int nitems,
-                                                            const PauliTermDev* __restrict__ terms,
-                                                            double* __restrict__ partial) {
-    __shared__ double sh[kEThreads];
-    // locate the item of this block (items sorted by block_first)
-    int lo = 0, hi = nitems - 1;
-    const int b = blockIdx.x;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (items[mid].block_first <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const ExpItem it = items[lo];
-    const int lb = b - it.block_first;
-    const V* s = psi + it.state_off;
-    const PauliTermDev* tm = terms + it.term_first;
-    const uint64_t stride = (uint64_t)it.nblocks * kEThreads;
-    double acc = 0.0;
-    if (it.x == 0 && (stride & (stride - 1)) == 0) {
-        // A thread visits i = i0 | u * stride (stride a power of two above
-        // i0), so a term's sign (-1)^{popc(i & zy)} splits into i0's part,
-        // folded into the coefficient once per term, and the step's part,
-        // the same for the whole wavefront (scalar unit).  Eight steps'
-        // probabilities stay in registers across the term chunks.
-        const uint64_t i0 = (uint64_t)lb * kEThreads + threadIdx.x;
-        for (uint64_t u0 = 0; i0 + u0 * stride < it.work; u0 += 8) {
-            double pv[8];
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const uint64_t i = i0 + (u0 + v) * stride;
-                pv[v] = 0.0;
-                if (i < it.work) {
-                    const V a = s[i];
-                    pv[v] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-                }
-            }
-            for (int q0 = 0; q0 < it.term_count; q0 += 8) {
-                const int nq = it.term_count - q0 < 8 ? it.term_count - q0 : 8;
-                double cp[8];
-                uint64_t zy[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    cp[j] = 0.0;
-                    zy[j] = 0;
-                    if (j < nq) {
-                        const PauliTermDev t = tm[q0 + j];
-                        zy[j] = t.zy;
-                        cp[j] = (__popcll(i0 & t.zy) & 1) ? -t.cr : t.cr;
-                    }
-                }
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    const uint64_t su = (u0 + v) * stride;
-                    double w = 0.0;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) w += (__popcll(su & zy[j]) & 1) ? -cp[j] : cp[j];
-                    acc += pv[v] * w;
-                }
-            }
-        }
-    } else if (it.x == 0) {
-        for (uint64_t i = (uint64_t)lb * kEThreads + threadIdx.x; i < it.work; i += stride) {
-            V a = s[i];
-            double p = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
-            double w = 0.0;
-            for (int t = 0; t < it.term_count; ++t) {
-                double c = tm[t].cr;
-                w += (__popcll(i & tm[t].zy) & 1) ? -c : c;
-            }
-            acc += p * w;
-        }
-    } else {
-        // Terms four at a time in registers.  For a pair (i, j = i ^ x) the
-        // two rows of term t add up to s_t(i) * 2 cr_t * Re(conj(a) b) when
-        // popc(x & zy_t) is even and s_t(i) * (-2 ci_t) * Im(conj(a) b) when
-        // it is odd, s_t(i) = (-1)^{popc(i & zy_t)}.
-        for (int q0 = 0; q0 < it.term_count; q0 += 4) {
-            const int nq = it.term_count - q0 < 4 ? it.term_count - q0 : 4;
-            uint64_t zy[4];
-            double gr[4], gi[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                zy[u] = 0;
-                gr[u] = gi[u] = 0.0;
-                if (u < nq) {
-                    const PauliTermDev q = tm[q0 + u];
-                    zy[u] = q.zy;
-                    if (__popcll(it.x & q.zy) & 1) gi[u] = -2.0 * q.ci;
-                    else gr[u] = 2.0 * q.cr;
-                }
-            }
-            for (uint64_t k = (uint64_t)lb * kEThreads + threadIdx.x; k < it.work; k += stride) {
-                const uint64_t i = insert_zero(k, it.h);
-                const V a = s[i], c2 = s[i ^ it.x];
-                const double zr = (double)a.x * (double)c2.x + (double)a.y * (double)c2.y;
-                const double zi = (double)a.x * (double)c2.y - (double)a.y * (double)c2.x;
-                double wr = 0.0, wi = 0.0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double sg = (__popcll(i & zy[u]) & 1) ? -1.0 : 1.0;
-                    wr += sg * gr[u];
-                    wi += sg * gi[u];
-                }
-                acc += zr * wr + zi * wi;
-            }
-        }
-    }
-    double t = bsum<kEThreads>(acc, sh);
-    if (threadIdx.x == 0) partial[b] = t;
 }
 
 // Every instance's op table becomes a copy of instance 0's (8-byte words).
@@ -293,7 +167,7 @@ struct BatchJob {
     const Plan* plan = nullptr;
     size_t nops = 0, goff = 0, plan_bytes = 0;
     int nitems = 0, blocks = 0;
-    size_t roff = 0, items_bytes = 0, terms_bytes = 0;
+    size_t roff = 0, items_bytes = 0, terms_bytes = 0, xoff = 0;
     // flip groups evaluated from LDS tiles (when every instance shares one
     // sum): layouts, their tables in the items buffer, partials after the
     // items' partials
@@ -352,15 +226,15 @@ static int job_ops(BatchJob& j, TileOp** out) {
     return LQ_OK;
 }
 
-// Expectation work items of the batch (obs_of[i] indexes `sums`).
+// Expectation work of the batch (obs_of[i] indexes `sums`): with one sum
+// shared by every instance, the flip groups that fit an LDS tile layout are
+// read from tiles; the other flip groups go into orbit items (up to
+// orbit_max_groups groups per read of a state) with each instance's diagonal
+// terms in its first item.  Items, per-instance block ranges and layout
+// tables go into the items buffer; orbit terms, then the X tables, into the
+// terms buffer.
 static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std::vector<int>& obs_of) {
-    std::vector<int> term_base(sums.size());
-    size_t nterms = 0;
-    for (size_t k = 0; k < sums.size(); ++k) {
-        term_base[k] = (int)nterms;
-        nterms += sums[k].terms.size();
-    }
-    std::vector<ExpItem> items;
+    std::vector<OrbitItem> items;
     std::vector<int> first(j.ninst), count(j.ninst);
     int blocks = 0;
     j.lays.clear();
@@ -375,39 +249,63 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
         for (int g : rest) tiled[g] = 0;
         if (!j.lays.empty()) j.tile_gx = (int)expect_tile_blocks(j.lays[0].geo, (uint64_t)j.ninst);
     }
+    // orbit sets of each distinct sum, their terms and X tables concatenated
+    const int maxm = orbit_max_groups(j.dtype);
+    std::vector<std::vector<OrbitSet>> sets(sums.size());
+    std::vector<int> term_base(sums.size()), x_base(sums.size());
+    size_t nterms = 0, nx = 0;
+    for (size_t k = 0; k < sums.size(); ++k) {
+        std::vector<int> flip;
+        int diag = -1;
+        for (size_t g = 0; g < sums[k].groups.size(); ++g) {
+            if (sums[k].groups[g].x == 0) diag = (int)g;
+            else if (tiled.empty() || !tiled[g]) flip.push_back((int)g);
+        }
+        plan_orbits(sums[k], flip, diag, maxm, sets[k]);
+        term_base[k] = (int)nterms;
+        x_base[k] = (int)nx;
+        for (const OrbitSet& o : sets[k]) {
+            nterms += o.terms.size();
+            nx += (size_t)1 << o.m;
+        }
+    }
     for (int i = 0; i < j.ninst; ++i) {
-        const PauliSum& ps = sums[obs_of[i]];
+        const int k = obs_of[i];
         first[i] = blocks;
-        for (const PauliGroup& g : ps.groups) {
-            if (!tiled.empty() && tiled[&g - ps.groups.data()]) continue;
-            ExpItem it;
+        int tb = term_base[k], xb = x_base[k];
+        for (const OrbitSet& o : sets[k]) {
+            OrbitItem it;
             std::memset(&it, 0, sizeof(it));
             it.state_off = (uint64_t)i * j.dim;
-            it.x = g.x;
-            it.work = g.x ? j.dim / 2 : j.dim;
-            it.h = g.x ? 63 - __builtin_clzll(g.x) : -1;
-            it.term_first = term_base[obs_of[i]] + g.first;
-            it.term_count = g.count;
+            it.work = j.dim >> o.m;
+            it.m = o.m;
+            for (int q = 0; q < o.m; ++q) it.piv[q] = o.piv[q];
+            it.x_first = xb;
+            it.term_first = tb;
+            it.term_count = (int)o.terms.size();
             it.block_first = blocks;
             it.nblocks = blocks_for(it.work);
             it.inst = i;
             blocks += it.nblocks;
+            tb += (int)o.terms.size();
+            xb += 1 << o.m;
             items.push_back(it);
         }
         count[i] = blocks - first[i];
     }
     j.nitems = (int)items.size();
     j.blocks = blocks;
-    const size_t ib = items.size() * sizeof(ExpItem);
-    const size_t rb = (size_t)j.ninst * 2 * sizeof(int);
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t ib = items.size() * sizeof(OrbitItem);
+    const size_t rb = (size_t)j.ninst * 2 * sizeof(int);
     j.roff = al(ib);
     j.items_bytes = al(j.roff + rb);
     for (const ExpLayout& l : j.lays) {
         j.lay_off.push_back(j.items_bytes);
         j.items_bytes += al(l.groups.size() * sizeof(ExpTileGroup)) + al(l.terms.size() * sizeof(ExpTileTerm));
     }
-    j.terms_bytes = std::max<size_t>(nterms, 1) * sizeof(PauliTermDev);
+    j.xoff = al(std::max<size_t>(nterms, 1) * sizeof(OrbitTerm));
+    j.terms_bytes = j.xoff + std::max<size_t>(nx, 1) * sizeof(uint64_t);
     j.tpart_off = (size_t)std::max(blocks, 1);
     const size_t tparts = j.lays.size() * (size_t)j.ninst * (size_t)j.tile_gx;
     int rc = g_items.get(j.items_bytes + 256);
@@ -428,10 +326,17 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
         std::memcpy(hi + j.lay_off[l] + al(lay.groups.size() * sizeof(ExpTileGroup)), lay.terms.data(),
                     lay.terms.size() * sizeof(ExpTileTerm));
     }
-    PauliTermDev* ht = (PauliTermDev*)g_hterms.p;
-    for (size_t k = 0; k < sums.size(); ++k)
-        if (!sums[k].terms.empty())
-            std::memcpy(ht + term_base[k], sums[k].terms.data(), sums[k].terms.size() * sizeof(PauliTermDev));
+    OrbitTerm* ht = (OrbitTerm*)g_hterms.p;
+    uint64_t* hx = (uint64_t*)((char*)g_hterms.p + j.xoff);
+    for (size_t k = 0; k < sums.size(); ++k) {
+        int tb = term_base[k], xb = x_base[k];
+        for (const OrbitSet& o : sets[k]) {
+            if (!o.terms.empty()) std::memcpy(ht + tb, o.terms.data(), o.terms.size() * sizeof(OrbitTerm));
+            std::memcpy(hx + xb, o.X, ((size_t)1 << o.m) * sizeof(uint64_t));
+            tb += (int)o.terms.size();
+            xb += 1 << o.m;
+        }
+    }
     return LQ_OK;
 }
 
@@ -520,14 +425,15 @@ static int job_launch(const BatchJob& j, void* psi) {
         const int* d_first = (const int*)((char*)g_items.p + j.roff);
         const int* d_count = d_first + j.ninst;
         if (j.blocks > 0) {
+            const uint64_t* d_x = (const uint64_t*)((const char*)g_terms.p + j.xoff);
             if (j.dtype == LQ_C128)
-                hipLaunchKernelGGL(k_expect_items<double2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
-                                   (const double2*)psi, (const ExpItem*)g_items.p, j.nitems,
-                                   (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+                hipLaunchKernelGGL(k_expect_orbits<double2>, dim3(j.blocks), dim3(kOrbitThreads), 0, stream(),
+                                   (const double2*)psi, (const OrbitItem*)g_items.p, j.nitems, d_x,
+                                   (const OrbitTerm*)g_terms.p, (double*)g_part.p);
             else
-                hipLaunchKernelGGL(k_expect_items<float2>, dim3(j.blocks), dim3(kEThreads), 0, stream(),
-                                   (const float2*)psi, (const ExpItem*)g_items.p, j.nitems,
-                                   (const PauliTermDev*)g_terms.p, (double*)g_part.p);
+                hipLaunchKernelGGL(k_expect_orbits<float2>, dim3(j.blocks), dim3(kOrbitThreads), 0, stream(),
+                                   (const float2*)psi, (const OrbitItem*)g_items.p, j.nitems, d_x,
+                                   (const OrbitTerm*)g_terms.p, (double*)g_part.p);
             LQ_LAUNCH_CHECK();
         }
         double* d_tpart = (double*)g_part.p + j.tpart_off;
@@ -566,7 +472,7 @@ static long g_graph_clock = 0;
 
 static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
     std::vector<uint64_t> k = {(uint64_t)j.n, (uint64_t)j.dtype, (uint64_t)j.ninst, j.nops, j.goff, j.plan_bytes,
-                               (uint64_t)j.nitems, (uint64_t)j.blocks, j.roff, j.items_bytes, j.terms_bytes,
+                               (uint64_t)j.nitems, (uint64_t)j.blocks, j.roff, j.items_bytes, j.terms_bytes, j.xoff,
                                (uint64_t)j.lays.size(), (uint64_t)j.tile_gx, j.tpart_off,
                                (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p, (uint64_t)g_items.p,
                                (uint64_t)g_hitems.p, (uint64_t)g_terms.p, (uint64_t)g_hterms.p, (uint64_t)g_part.p,

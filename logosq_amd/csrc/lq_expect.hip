@@ -105,59 +105,6 @@ __global__ void __launch_bounds__(kExpThreads) k_expect_diag(const V* __restrict
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x != 0: pairs {i, j = i ^ x} with bit h (top bit of x) of i clear.  A
-// wavefront takes 64 consecutive pair indices k (i = k with a zero inserted
-// at bit h), U chunks of them in flight.
-template <typename V>
-__global__ void __launch_bounds__(kExpThreads) k_expect_flip(const V* __restrict__ psi, uint64_t chunks,
-                                                             uint32_t lane_limit, uint64_t x, int h,
-                                                             const PauliTermDev* __restrict__ terms, int nt,
-                                                             double* __restrict__ partial) {
-    __shared__ double sh[kExpThreads];
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t w0 = (uint64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * (kExpThreads / 64) +
-                                                                 (threadIdx.x >> 6));
-    const uint64_t W = (uint64_t)gridDim.x * (kExpThreads / 64);
-    constexpr int U = 4;
-    double acc = 0.0;
-    if (lane < lane_limit) {
-        for (uint64_t c = w0; c < chunks; c += U * W) {
-            V a[U], b[U];
-            uint64_t ii[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                ii[u] = insert_zero(((c + u * W) << 6) | lane, h);
-                if (c + u * W < chunks) {
-                    a[u] = psi[ii[u]];
-                    b[u] = psi[ii[u] ^ x];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (c + u * W >= chunks) break;
-                const uint64_t i = ii[u], j = i ^ x;
-                // z = conj(a) * b
-                const double zr = (double)a[u].x * (double)b[u].x + (double)a[u].y * (double)b[u].y;
-                const double zi = (double)a[u].x * (double)b[u].y - (double)a[u].y * (double)b[u].x;
-                double wir = 0.0, wii = 0.0, wjr = 0.0, wji = 0.0;
-                for (int t = 0; t < nt; ++t) {
-                    const PauliTermDev tm = terms[t];
-                    const bool si = __popcll(i & tm.zy) & 1;
-                    const bool sj = __popcll(j & tm.zy) & 1;
-                    wir += si ? -tm.cr : tm.cr;
-                    wii += si ? -tm.ci : tm.ci;
-                    wjr += sj ? -tm.cr : tm.cr;
-                    wji += sj ? -tm.ci : tm.ci;
-                }
-                // Re(z * w_i) + Re(conj(z) * w_j)
-                acc += zr * wir - zi * wii + zr * wjr + zi * wji;
-            }
-        }
-    }
-    double s = block_sum_e<kExpThreads>(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
 // ---- flip groups from LDS tiles ------------------------------------------
 // A tile of 2^K amplitudes (the L lowest bits and hb chosen high bits, as in
 // the fused engine's passes) is read into LDS once, and every flip group
@@ -484,6 +431,96 @@ int compile_pauli_sum(int n, const uint8_t* paulis, const double* coeffs, int nu
     return LQ_OK;
 }
 
+int orbit_max_groups(int dtype) {
+    (void)dtype;
+    return kOrbitMax;
+}
+
+void plan_orbits(const PauliSum& ps, const std::vector<int>& groups, int diag, int maxm, std::vector<OrbitSet>& out) {
+    out.clear();
+    maxm = std::max(1, std::min(maxm, kOrbitMax));
+    // per set: a reduced basis (each vector has exactly one of the set's
+    // pivot bits), its pivots, and the member groups (the generators)
+    struct Build {
+        std::vector<uint64_t> basis;
+        std::vector<int> pivb, gens;
+    };
+    std::vector<Build> builds;
+    for (int g : groups) {
+        const uint64_t x = ps.groups[g].x;
+        bool placed = false;
+        for (Build& b : builds) {
+            if ((int)b.gens.size() >= maxm) continue;
+            uint64_t r = x;
+            for (size_t k = 0; k < b.basis.size(); ++k)
+                if ((r >> b.pivb[k]) & 1ull) r ^= b.basis[k];
+            if (!r) continue;  // x lies in the set's span
+            const int p = 63 - __builtin_clzll(r);
+            for (uint64_t& v : b.basis)
+                if ((v >> p) & 1ull) v ^= r;
+            b.basis.push_back(r);
+            b.pivb.push_back(p);
+            b.gens.push_back(g);
+            placed = true;
+            break;
+        }
+        if (!placed) {
+            Build b;
+            b.basis.push_back(x);
+            b.pivb.push_back(63 - __builtin_clzll(x));
+            b.gens.push_back(g);
+            builds.push_back(b);
+        }
+    }
+    auto sigma = [](const OrbitSet& o, uint64_t zy) {
+        uint32_t s = 0;
+        for (int c = 0; c < (1 << o.m); ++c) s |= (uint32_t)(__builtin_popcountll(o.X[c] & zy) & 1) << c;
+        return s;
+    };
+    for (const Build& b : builds) {
+        OrbitSet o;
+        o.m = (int)b.gens.size();
+        std::vector<int> pv = b.pivb;
+        std::sort(pv.begin(), pv.end());
+        for (int j = 0; j < o.m; ++j) o.piv[j] = pv[j];
+        for (int c = 0; c < (1 << o.m); ++c) {
+            uint64_t xc = 0;
+            for (int j = 0; j < o.m; ++j)
+                if ((c >> j) & 1) xc ^= ps.groups[b.gens[j]].x;
+            o.X[c] = xc;
+        }
+        for (int j = 0; j < o.m; ++j) {
+            const PauliGroup& grp = ps.groups[b.gens[j]];
+            for (int t = grp.first; t < grp.first + grp.count; ++t) {
+                const PauliTermDev& d = ps.terms[t];
+                OrbitTerm q;
+                std::memset(&q, 0, sizeof(q));
+                q.zy = d.zy;
+                q.gen = j;
+                q.imag = __builtin_popcountll(grp.x & d.zy) & 1;
+                q.g = q.imag ? -2.0 * d.ci : 2.0 * d.cr;
+                q.sigma = sigma(o, d.zy);
+                o.terms.push_back(q);
+            }
+        }
+        out.push_back(o);
+    }
+    if (diag >= 0 && ps.groups[diag].count > 0) {
+        if (out.empty()) out.emplace_back();
+        OrbitSet& o = out[0];
+        const PauliGroup& grp = ps.groups[diag];
+        for (int t = grp.first; t < grp.first + grp.count; ++t) {
+            OrbitTerm q;
+            std::memset(&q, 0, sizeof(q));
+            q.zy = ps.terms[t].zy;
+            q.gen = -1;
+            q.g = ps.terms[t].cr;
+            q.sigma = sigma(o, q.zy);
+            o.terms.push_back(q);
+        }
+    }
+}
+
 // Device memory an expectation works in when the caller supplies it (the
 // no-allocation entry point): its base and size.  Null: the library's
 // growable scratch and pinned staging.
@@ -508,19 +545,15 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     const uint64_t items = chunks / R;
     const uint64_t rmask = (uint64_t)(R - 1) << 6;
     int nb_diag = (int)std::min<uint64_t>(std::max<uint64_t>((items + 3) / 4, 1), kExpBlocksMax);
-    const uint64_t half = st->dim >> 1;
-    const uint64_t fchunks = half >= 64 ? half >> 6 : 1;
-    const uint32_t flanes = half >= 64 ? 64u : (uint32_t)half;
-    int nb_flip = (int)std::min<uint64_t>(std::max<uint64_t>((fchunks + 15) / 16, 1), kExpBlocksMax);
-    int nb = std::max(nb_diag, nb_flip);
     // Device term table: the diagonal group's terms are reordered static
     // first (masks that miss the r bits), then dynamic; a launch takes every
     // static term and up to kDiagDyn dynamic ones, further launches the rest.
     std::vector<PauliTermDev> dev = ps.terms;
     int diag_first = -1, diag_static = 0, diag_count = 0;
     size_t launches = 0;
-    // flip groups: from LDS tiles where they fit a layout, else pair kernels;
-    // the run-time specialised layouts when the policy allows and all compile
+    // flip groups: from LDS tiles where they fit a layout, else orbit items
+    // (several groups per read of the state); the run-time specialised
+    // layouts when the policy allows and all compile
     std::vector<ExpLayout> lays;
     std::vector<int> rest;
     XPlan xp;
@@ -543,12 +576,32 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
         lay_blocks.push_back(expect_tile_blocks(l.geo, 1));
         tile_parts += lay_blocks.back();
     }
+    std::vector<int> pair_groups;
+    for (int g = 0; g < ng; ++g)
+        if (paired[g] && ps.groups[g].x != 0) pair_groups.push_back(g);
+    std::vector<OrbitSet> osets;
+    plan_orbits(ps, pair_groups, -1, orbit_max_groups(st->dtype), osets);
+    std::vector<OrbitItem> oitems;
+    size_t oterms = 0, ox = 0;
+    int oblocks = 0;
+    for (const OrbitSet& o : osets) {
+        OrbitItem it;
+        std::memset(&it, 0, sizeof(it));
+        it.work = st->dim >> o.m;
+        it.m = o.m;
+        for (int q = 0; q < o.m; ++q) it.piv[q] = o.piv[q];
+        it.x_first = (int)ox;
+        it.term_first = (int)oterms;
+        it.term_count = (int)o.terms.size();
+        it.block_first = oblocks;
+        it.nblocks = (int)std::min<uint64_t>(std::max<uint64_t>((it.work + 1023) / 1024, 1), kExpBlocksMax);
+        oblocks += it.nblocks;
+        oterms += o.terms.size();
+        ox += (size_t)1 << o.m;
+        oitems.push_back(it);
+    }
     for (const PauliGroup& grp : ps.groups) {
-        if (grp.x != 0) {
-            if (paired[&grp - ps.groups.data()]) ++launches;
-            continue;
-        }
-        if (diag_in_tiles) continue;
+        if (grp.x != 0 || diag_in_tiles) continue;
         std::vector<PauliTermDev> st_, dy_;
         for (int t = grp.first; t < grp.first + grp.count; ++t)
             ((ps.terms[t].zy & rmask) == 0 ? st_ : dy_).push_back(ps.terms[t]);
@@ -569,7 +622,14 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     }
     const size_t x_coef_off = lay_bytes;  // the specialised layouts' coefficient table
     if (use_x) lay_bytes += al(expect_jit_coef_count(xp) * sizeof(double));
-    size_t part_count = launches * nb + tile_parts;
+    // orbit tables: [items][terms][X]
+    const size_t o_items_off = lay_bytes;
+    lay_bytes += al(oitems.size() * sizeof(OrbitItem));
+    const size_t o_terms_off = lay_bytes;
+    lay_bytes += al(oterms * sizeof(OrbitTerm));
+    const size_t o_x_off = lay_bytes;
+    lay_bytes += al(ox * sizeof(uint64_t));
+    size_t part_count = launches * nb_diag + tile_parts + oblocks;
     const size_t need = term_pad + lay_bytes + (part_count + 8) * sizeof(double);
     char* base;
     double* h;
@@ -598,6 +658,17 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
     double* d_out = d_part + part_count;
     std::memcpy(h, dev.data(), term_bytes);
     if (use_x) expect_jit_coefs(xp, (double*)((char*)h + term_pad + x_coef_off));
+    if (!oitems.empty()) {
+        char* ho = (char*)h + term_pad;
+        std::memcpy(ho + o_items_off, oitems.data(), oitems.size() * sizeof(OrbitItem));
+        size_t tb = 0, xb = 0;
+        for (const OrbitSet& o : osets) {
+            std::memcpy(ho + o_terms_off + tb * sizeof(OrbitTerm), o.terms.data(), o.terms.size() * sizeof(OrbitTerm));
+            std::memcpy(ho + o_x_off + xb * sizeof(uint64_t), o.X, ((size_t)1 << o.m) * sizeof(uint64_t));
+            tb += o.terms.size();
+            xb += (size_t)1 << o.m;
+        }
+    }
     for (size_t li = 0; li < lays.size(); ++li) {
         char* hp = (char*)h + term_pad + lay_off[li];
         const ExpLayout& l = lays[li];
@@ -640,14 +711,16 @@ static int expect_t(const lq_state* st, const PauliSum& ps, double* out, const E
                 written += nb_diag;
                 t0 += kDiagDyn;
             } while (t0 < nd_all);
-        } else if (paired[g]) {
-            int hbit = 63 - __builtin_clzll(grp.x);
-            hipLaunchKernelGGL(k_expect_flip<V>, dim3(nb_flip), dim3(kExpThreads), 0, stream(),
-                               (const V*)st->d_amps, fchunks, flanes, grp.x, hbit,
-                               (const PauliTermDev*)(d_terms + grp.first), grp.count, d_part + written);
-            written += nb_flip;
         }
         LQ_LAUNCH_CHECK();
+    }
+    if (oblocks > 0) {
+        hipLaunchKernelGGL(k_expect_orbits<V>, dim3(oblocks), dim3(kOrbitThreads), 0, stream(), (const V*)st->d_amps,
+                           (const OrbitItem*)(d_lay + o_items_off), (int)oitems.size(),
+                           (const uint64_t*)(d_lay + o_x_off), (const OrbitTerm*)(d_lay + o_terms_off),
+                           d_part + written);
+        LQ_LAUNCH_CHECK();
+        written += oblocks;
     }
     hipLaunchKernelGGL(k_expect_final, dim3(1), dim3(1024), 0, stream(), (const double*)d_part, written, d_out);
     LQ_LAUNCH_CHECK();
@@ -725,11 +798,20 @@ extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, 
     if (rc) return rc;
     int diag = 0;
     for (const PauliGroup& g : ps.groups) diag += g.x == 0;
+    // reads for the flip groups outside the tiles: one per orbit item
+    auto orbit_reads = [&](const std::vector<int>& rest) {
+        std::vector<int> flip;
+        for (int g : rest)
+            if (ps.groups[g].x != 0) flip.push_back(g);
+        std::vector<OrbitSet> sets;
+        plan_orbits(ps, flip, -1, orbit_max_groups(dtype), sets);
+        return (int)sets.size();
+    };
     XPlan xp;
     if (expect_jit_wanted(num_qubits, dtype) && plan_expect_jit(num_qubits, dtype, ps, xp)) {
         // the run-time specialised layouts (used once their kernels compile)
         if (tile_layouts) *tile_layouts = (int)xp.layouts.size();
-        if (pair_groups) *pair_groups = (int)xp.rest.size();
+        if (pair_groups) *pair_groups = orbit_reads(xp.rest);
         if (diag_groups) *diag_groups = xp.diag_in ? 0 : diag;
         return LQ_OK;
     }
@@ -737,7 +819,7 @@ extern "C" int lq_expect_plan(int num_qubits, int dtype, const uint8_t* paulis, 
     std::vector<int> rest;
     plan_expect_tiles(num_qubits, dtype, ps, lays, rest);
     if (tile_layouts) *tile_layouts = (int)lays.size();
-    if (pair_groups) *pair_groups = (int)rest.size();
+    if (pair_groups) *pair_groups = orbit_reads(rest);
     if (diag_groups) *diag_groups = diag;
     return LQ_OK;
 }

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "lq_common.h"
+#include "lq_orbit.h"
 
 namespace lq {
 
@@ -36,6 +37,21 @@ struct PauliSum {
 
 int compile_pauli_sum(int n, const uint8_t* paulis, const double* coeffs, int num_terms, int convention,
                       PauliSum* out);
+
+// Orbit items (lq_orbit.h): flip groups `groups` (indices into ps.groups,
+// non-zero masks) packed first-fit into sets of at most maxm linearly
+// independent masks, each evaluated from one read of the state; with
+// diag >= 0 that group's terms (x = 0) join the first set (a set with m = 0
+// when there is no flip group).
+struct OrbitSet {
+    int m = 0;
+    int piv[kOrbitMax] = {};
+    uint64_t X[1 << kOrbitMax] = {};
+    std::vector<OrbitTerm> terms;
+};
+void plan_orbits(const PauliSum& ps, const std::vector<int>& groups, int diag, int maxm, std::vector<OrbitSet>& out);
+// Groups per orbit item for a dtype: 4 (16 amplitudes per thread) for both.
+int orbit_max_groups(int dtype);
 
 // Flip groups evaluated from LDS tiles: a layout is a tile geometry (the L
 // lowest bits plus hb high bits) and the flip groups whose masks lie in it.

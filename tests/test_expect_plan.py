@@ -1,5 +1,6 @@
 """The expectation planner (host code, no GPU): which flip groups are read
-from LDS tile layouts, which alone, in each mode.  Generic tile kernels
+from LDS tile layouts and how many reads the others take (orbit items: up
+to four linearly independent flip masks per read), in each mode.  Generic tile kernels
 (run-time specialisation off): a layout holds the 5 lowest bits and 6
 (complex128) / 7 (complex64) more, so a ring's bonds pack about 6 to a
 layout.  Specialised kernels (the default): 128 KiB tiles whose low bits may
@@ -42,9 +43,9 @@ def test_ring_30_specialised_three_reads(tiles_mode):
     assert plan(30, codes, coeffs, abi.LQ_C64) == (4, 0, 0)
     assert plan(20, codes[:0], coeffs[:0]) == (0, 0, 0)
     small = algorithms.heisenberg_codes(20, 0.7, -0.4, 0.9)
-    assert plan(20, *small) == (0, 20, 1)  # 16 MiB: cache-resident, one read per group
+    assert plan(20, *small) == (0, 5, 1)  # 16 MiB: no tiles; 20 bonds, four per read
     tiles_mode(abi.EXPECT_TILES_OFF)
-    assert plan(30, codes, coeffs) == (0, 30, 1)
+    assert plan(30, codes, coeffs) == (0, 8, 1)  # 30 bonds, four per read
 
 
 def test_ring_30_auto_uses_tiles(tiles_mode):
@@ -56,9 +57,9 @@ def test_ring_30_auto_uses_tiles(tiles_mode):
     assert pair64 == 0 and lay64 <= lay
 
 
-def test_small_state_auto_reads_per_group(tiles_mode):
+def test_small_state_auto_orbit_reads(tiles_mode):
     codes, coeffs = algorithms.heisenberg_codes(20, 0.7, -0.4, 0.9)
-    assert plan(20, codes, coeffs) == (0, 20, 1)  # 16 MiB: cache-resident, one read per group
+    assert plan(20, codes, coeffs) == (0, 5, 1)  # 16 MiB: no tiles; 20 bonds, four per read
 
 
 def test_modes_and_wide_strings(tiles_mode):
@@ -72,9 +73,9 @@ def test_modes_and_wide_strings(tiles_mode):
     lay, pair, diag = plan(n, codes, coeffs)
     assert lay >= 1 and pair == 1 and diag == 1
     tiles_mode(abi.EXPECT_TILES_OFF)
-    assert plan(n, codes, coeffs) == (0, n + 1, 1)
+    assert plan(n, codes, coeffs) == (0, 4, 1)  # 14 bonds and the wide string, four per read
     tiles_mode(abi.EXPECT_TILES_ON)
-    assert plan(7, *algorithms.heisenberg_codes(7, 1, 1, 1)) == (0, 7, 1)  # under 8 qubits: pair kernel
+    assert plan(7, *algorithms.heisenberg_codes(7, 1, 1, 1)) == (0, 2, 1)  # under 8 qubits: no tiles
     jit.set_jit(jit.REPEAT)  # specialised: the diagonal group joins the first layout
     lay, pair, diag = plan(n, codes, coeffs)
     assert lay >= 1 and pair == 1 and diag == 0
