@@ -7,9 +7,16 @@
 //   - batches of unrelated circuits of uneven qubit counts.
 // Accumulation is f64 for both amplitude types, in a fixed order.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "lq_fused.h"
@@ -109,6 +116,21 @@ __global__ void __launch_bounds__(64) k_apply_overrides(TileOp* __restrict__ ops
     if (threadIdx.x < 32) ops[(uint64_t)o.inst * nops + o.op].m[threadIdx.x] = o.m[threadIdx.x];
 }
 
+// Op tables from a base table and per-instance matrices: instance i's op o
+// is base[o] with m[0..7] = m8[i][o][0..7] (pair and diagonal ops carry
+// their 2x2 matrix / four phases there), in 8-byte words.
+static_assert(offsetof(TileOp, m) % 8 == 0 && sizeof(TileOp) % 8 == 0, "TileOp words");
+__global__ void __launch_bounds__(256) k_expand_ops(uint64_t* __restrict__ dst, const uint64_t* __restrict__ base,
+                                                    const uint64_t* __restrict__ m8, uint64_t nops,
+                                                    uint64_t count) {
+    constexpr uint64_t W = sizeof(TileOp) / 8, M0 = offsetof(TileOp, m) / 8;
+    const uint64_t total = W * nops * count;
+    for (uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (uint64_t)gridDim.x * 256) {
+        const uint64_t w = t % W, io = t / W, o = io % nops;
+        dst[t] = (w >= M0 && w < M0 + 8) ? m8[io * 8 + (w - M0)] : base[o * W + w];
+    }
+}
+
 // States first .. first+count-1 of the batch become copies of state 0.
 template <typename V>
 __global__ void __launch_bounds__(256) k_copy_base(V* __restrict__ psi, uint64_t dim, uint64_t first, uint64_t count) {
@@ -184,6 +206,9 @@ struct BatchJob {
     // table into every instance's slot and applies the overrides
     bool replicate = false;
     size_t nover = 0, hgoff = 0, hover = 0, hbytes = 0, stage = 0;
+    // instances of uneven sizes (expectation only): instance i's amplitude
+    // offset and size; empty: i * dim and dim
+    std::vector<uint64_t> inst_off, inst_dim;
 };
 
 
@@ -226,6 +251,23 @@ static int job_ops(BatchJob& j, TileOp** out) {
     return LQ_OK;
 }
 
+// Run fn(i) for i in [0, count) on up to 8 host threads (inline when small).
+template <typename F>
+static void parallel_for(int count, F&& fn) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min<unsigned>({8u, hw, (unsigned)std::max(count / 16, 1)});
+    if (nth <= 1) {
+        for (int i = 0; i < count; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+        th.emplace_back([&, t] {
+            for (int i = t; i < count; i += nth) fn(i);
+        });
+    for (auto& x : th) x.join();
+}
+
 // Expectation work of the batch (obs_of[i] indexes `sums`): with one sum
 // shared by every instance, the flip groups that fit an LDS tile layout are
 // read from tiles; the other flip groups go into orbit items (up to
@@ -241,7 +283,7 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     j.lay_off.clear();
     j.tile_gx = 0;
     std::vector<char> tiled;
-    if (sums.size() == 1) {
+    if (sums.size() == 1 && j.inst_off.empty()) {
         std::vector<int> rest;
         plan_expect_tiles(j.n, j.dtype, sums[0], j.lays, rest);
         tiled.assign(sums[0].groups.size(), 0);
@@ -254,7 +296,7 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     std::vector<std::vector<OrbitSet>> sets(sums.size());
     std::vector<int> term_base(sums.size()), x_base(sums.size());
     size_t nterms = 0, nx = 0;
-    for (size_t k = 0; k < sums.size(); ++k) {
+    parallel_for((int)sums.size(), [&](int k) {
         std::vector<int> flip;
         int diag = -1;
         for (size_t g = 0; g < sums[k].groups.size(); ++g) {
@@ -262,6 +304,8 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
             else if (tiled.empty() || !tiled[g]) flip.push_back((int)g);
         }
         plan_orbits(sums[k], flip, diag, maxm, sets[k]);
+    });
+    for (size_t k = 0; k < sums.size(); ++k) {
         term_base[k] = (int)nterms;
         x_base[k] = (int)nx;
         for (const OrbitSet& o : sets[k]) {
@@ -276,8 +320,8 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
         for (const OrbitSet& o : sets[k]) {
             OrbitItem it;
             std::memset(&it, 0, sizeof(it));
-            it.state_off = (uint64_t)i * j.dim;
-            it.work = j.dim >> o.m;
+            it.state_off = j.inst_off.empty() ? (uint64_t)i * j.dim : j.inst_off[i];
+            it.work = (j.inst_dim.empty() ? j.dim : j.inst_dim[i]) >> o.m;
             it.m = o.m;
             for (int q = 0; q < o.m; ++q) it.piv[q] = o.piv[q];
             it.x_first = xb;
@@ -356,6 +400,8 @@ static int copy_base(const BatchJob& j, void* psi, int first, int count) {
     return LQ_OK;
 }
 
+static int expect_launch(const BatchJob& j, void* psi);
+
 static int job_launch(const BatchJob& j, void* psi) {
     const size_t nsteps = j.plan->steps.size();
     auto alive = [&](size_t k) { return j.alive.empty() ? j.ninst : j.alive[k]; };
@@ -419,6 +465,12 @@ static int job_launch(const BatchJob& j, void* psi) {
         int rc = copy_base(j, psi, alive(nsteps - 1), j.ninst - alive(nsteps - 1));
         if (rc) return rc;
     }
+    return expect_launch(j, psi);
+}
+
+// The expectation part of a job's launch sequence: item and term upload,
+// orbit items, tile layouts, per-instance sums, result copy to g_hout.
+static int expect_launch(const BatchJob& j, void* psi) {
     if (j.blocks > 0 || !j.lays.empty()) {
         LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, g_hitems.p, j.items_bytes, hipMemcpyHostToDevice, stream()));
         LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, g_hterms.p, j.terms_bytes, hipMemcpyHostToDevice, stream()));
@@ -470,96 +522,108 @@ static std::vector<GraphEntry> g_graphs;
 static std::vector<std::vector<uint64_t>> g_seen_keys;
 static long g_graph_clock = 0;
 
-static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
-    std::vector<uint64_t> k = {(uint64_t)j.n, (uint64_t)j.dtype, (uint64_t)j.ninst, j.nops, j.goff, j.plan_bytes,
-                               (uint64_t)j.nitems, (uint64_t)j.blocks, j.roff, j.items_bytes, j.terms_bytes, j.xoff,
-                               (uint64_t)j.lays.size(), (uint64_t)j.tile_gx, j.tpart_off,
-                               (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p, (uint64_t)g_items.p,
-                               (uint64_t)g_hitems.p, (uint64_t)g_terms.p, (uint64_t)g_hterms.p, (uint64_t)g_part.p,
-                               (uint64_t)g_out.p, (uint64_t)g_hout.p, j.plan->steps.size()};
-    for (const Step& s : j.plan->steps) {
-        const TilePass& p = s.pass;
-        k.push_back(s.normalize ? 1 : 0);
-        k.push_back(((uint64_t)(uint32_t)p.L << 32) | (uint32_t)p.hb);
-        for (int h = 0; h < p.hb; ++h) k.push_back((uint64_t)p.hpos[h]);
-        k.push_back(((uint64_t)(uint32_t)p.op_first << 32) | (uint32_t)p.op_count);
-        k.push_back(((uint64_t)(uint32_t)p.group_first << 32) | (uint32_t)p.group_count);
-        k.push_back(((uint64_t)(uint32_t)p.rich << 32));
-        k.push_back(p.ntiles);
-        k.push_back((uint64_t)(uintptr_t)s.jit);
+// Graph-key parts: a tile pass step (every launch argument it implies) and
+// the expectation part of a job (sizes and buffer addresses).
+static void step_key(std::vector<uint64_t>& k, const Step& s) {
+    const TilePass& p = s.pass;
+    k.push_back(s.normalize ? 1 : 0);
+    k.push_back(((uint64_t)(uint32_t)p.L << 32) | (uint32_t)p.hb);
+    for (int h = 0; h < p.hb; ++h) k.push_back((uint64_t)p.hpos[h]);
+    k.push_back(((uint64_t)(uint32_t)p.op_first << 32) | (uint32_t)p.op_count);
+    k.push_back(((uint64_t)(uint32_t)p.group_first << 32) | (uint32_t)p.group_count);
+    k.push_back(((uint64_t)(uint32_t)p.rich << 32));
+    k.push_back(p.ntiles);
+    k.push_back((uint64_t)(uintptr_t)s.jit);
+}
+
+static void expect_key(std::vector<uint64_t>& k, const BatchJob& j) {
+    for (uint64_t v : {(uint64_t)j.ninst, (uint64_t)j.nitems, (uint64_t)j.blocks, (uint64_t)j.roff,
+                       (uint64_t)j.items_bytes, (uint64_t)j.terms_bytes, (uint64_t)j.xoff, (uint64_t)j.lays.size(),
+                       (uint64_t)j.tile_gx, (uint64_t)j.tpart_off, (uint64_t)g_items.p, (uint64_t)g_hitems.p,
+                       (uint64_t)g_terms.p, (uint64_t)g_hterms.p, (uint64_t)g_part.p, (uint64_t)g_out.p,
+                       (uint64_t)g_hout.p})
+        k.push_back(v);
+    for (const ExpLayout& l : j.lays) {
+        k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
+        for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
+        k.push_back(((uint64_t)l.groups.size() << 32) | l.terms.size());
     }
+}
+
+static std::vector<uint64_t> job_key(const BatchJob& j, const void* psi) {
+    std::vector<uint64_t> k = {(uint64_t)j.n, (uint64_t)j.dtype, j.nops, j.goff, j.plan_bytes,
+                               (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p, j.plan->steps.size()};
+    for (const Step& s : j.plan->steps) step_key(k, s);
     k.push_back(j.alive.size());
     for (int a : j.alive) k.push_back((uint64_t)a);
     k.push_back(j.replicate ? 1 : 0);
     k.push_back(j.nover);
     k.push_back(j.hbytes);
     k.push_back(j.stage);
-    for (const ExpLayout& l : j.lays) {
-        k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
-        for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
-        k.push_back(((uint64_t)l.groups.size() << 32) | l.terms.size());
-    }
+    expect_key(k, j);
     return k;
+}
+
+// Run a launch sequence through the graph cache: a key seen for the first
+// time runs eagerly, the second time its launches are captured into a graph,
+// later it replays.  graphable=false: always eager.
+template <typename F>
+static int run_graphed(const std::vector<uint64_t>& key, bool graphable, F&& launch) {
+    if (!graphable || !g_graphs_on) return launch();
+    GraphEntry* hit = nullptr;
+    for (GraphEntry& e : g_graphs)
+        if (e.key == key) hit = &e;
+    if (!hit && std::find(g_seen_keys.begin(), g_seen_keys.end(), key) != g_seen_keys.end()) {
+        // second sighting: capture the launch sequence
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+        LQ_HIP_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        int rc = launch();
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(stream(), &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            set_error("graph capture failed: %s", hipGetErrorString(e));
+            return LQ_ERR_DEVICE;
+        }
+        GraphEntry ne;
+        ne.key = key;
+        e = hipGraphInstantiate(&ne.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            set_error("graph instantiation failed: %s", hipGetErrorString(e));
+            return LQ_ERR_DEVICE;
+        }
+        if (g_graphs.size() >= 16) {
+            auto old = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                        [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+            (void)hipGraphExecDestroy(old->exec);
+            g_graphs.erase(old);
+        }
+        g_graphs.push_back(ne);
+        hit = &g_graphs.back();
+        ++g_graph_captures;
+    }
+    if (hit) {
+        hit->last_use = ++g_graph_clock;
+        LQ_HIP_CHECK(hipGraphLaunch(hit->exec, stream()));
+        ++g_graph_replays;
+        return LQ_OK;
+    }
+    g_seen_keys.push_back(key);
+    if (g_seen_keys.size() > 64) g_seen_keys.erase(g_seen_keys.begin());
+    return launch();
 }
 
 // Run a prepared job and wait for its energies (in g_hout).
 static int job_run(const BatchJob& j, void* psi, double* energies) {
-    bool graphable = g_graphs_on;
+    bool graphable = true;
     for (const Step& s : j.plan->steps)
         if (s.normalize) graphable = false;
-    int rc = LQ_OK;
-    if (graphable) {
-        std::vector<uint64_t> key = job_key(j, psi);
-        GraphEntry* hit = nullptr;
-        for (GraphEntry& e : g_graphs)
-            if (e.key == key) hit = &e;
-        if (!hit && std::find(g_seen_keys.begin(), g_seen_keys.end(), key) != g_seen_keys.end()) {
-            // second sighting: capture the launch sequence
-            LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-            LQ_HIP_CHECK(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-            rc = job_launch(j, psi);
-            hipGraph_t graph = nullptr;
-            hipError_t e = hipStreamEndCapture(stream(), &graph);
-            if (rc) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return rc;
-            }
-            if (e != hipSuccess) {
-                set_error("graph capture failed: %s", hipGetErrorString(e));
-                return LQ_ERR_DEVICE;
-            }
-            GraphEntry ne;
-            ne.key = key;
-            e = hipGraphInstantiate(&ne.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) {
-                set_error("graph instantiation failed: %s", hipGetErrorString(e));
-                return LQ_ERR_DEVICE;
-            }
-            if (g_graphs.size() >= 16) {
-                auto old = std::min_element(g_graphs.begin(), g_graphs.end(), [](const GraphEntry& a,
-                                                                                 const GraphEntry& b) {
-                    return a.last_use < b.last_use;
-                });
-                (void)hipGraphExecDestroy(old->exec);
-                g_graphs.erase(old);
-            }
-            g_graphs.push_back(ne);
-            hit = &g_graphs.back();
-            ++g_graph_captures;
-        }
-        if (hit) {
-            hit->last_use = ++g_graph_clock;
-            LQ_HIP_CHECK(hipGraphLaunch(hit->exec, stream()));
-            ++g_graph_replays;
-        } else {
-            g_seen_keys.push_back(key);
-            if (g_seen_keys.size() > 64) g_seen_keys.erase(g_seen_keys.begin());
-            rc = job_launch(j, psi);
-        }
-    } else {
-        rc = job_launch(j, psi);
-    }
+    int rc = run_graphed(graphable && g_graphs_on ? job_key(j, psi) : std::vector<uint64_t>(), graphable,
+                         [&] { return job_launch(j, psi); });
     if (rc) return rc;
     LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     if (j.blocks > 0 || !j.lays.empty()) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
@@ -744,27 +808,334 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
     return LQ_OK;
 }
 
+// The values a circuit's structure key holds, one by one: kinds, qubits,
+// flags (angles excluded); dense matrix gates add their position.
+template <typename F>
+static void structure_walk(int n, const lq_gate* g, int count, F&& put) {
+    put(n);
+    put(count);
+    for (int i = 0; i < count; ++i) {
+        put(g[i].kind);
+        put(g[i].qubits[0]);
+        put(g[i].qubits[1]);
+        put(g[i].qubits[2]);
+        put(g[i].flags);
+        if (g[i].kind == LQ_GATE_MATRIX1 || g[i].kind == LQ_GATE_MATRIX2) put(-1000 - i);
+    }
+}
+
+static uint64_t structure_hash(int n, const lq_gate* g, int count) {
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    structure_walk(n, g, count, [&](int64_t v) {
+        h ^= (uint64_t)v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+        h *= 0xbf58476d1ce4e5b9ull;
+    });
+    return h;
+}
+
+static bool same_structure(int na, const lq_gate* a, int ca, int nb, const lq_gate* b, int cb) {
+    if (na != nb || ca != cb) return false;
+    for (int i = 0; i < ca; ++i) {
+        if (a[i].kind != b[i].kind || a[i].flags != b[i].flags || a[i].qubits[0] != b[i].qubits[0] ||
+            a[i].qubits[1] != b[i].qubits[1] || a[i].qubits[2] != b[i].qubits[2])
+            return false;
+    }
+    return true;
+}
+
 // Structure key of a circuit: kinds, qubits, flags (angles excluded).
 static std::vector<int64_t> structure_key(int n, const lq_gate* g, int count) {
     std::vector<int64_t> k;
     k.reserve(2 + (size_t)count * 5);
-    k.push_back(n);
-    k.push_back(count);
-    for (int i = 0; i < count; ++i) {
-        k.push_back(g[i].kind);
-        k.push_back(g[i].qubits[0]);
-        k.push_back(g[i].qubits[1]);
-        k.push_back(g[i].qubits[2]);
-        k.push_back(g[i].flags);
-        if (g[i].kind == LQ_GATE_MATRIX1 || g[i].kind == LQ_GATE_MATRIX2) k.push_back(-1000 - i);  // no sharing
-    }
+    structure_walk(n, g, count, [&](int64_t v) { k.push_back(v); });
     return k;
+}
+
+// ---------------------------------------------------------------------------------
+// Uneven batches: circuits of several structures (qubit counts, gate layouts)
+// in one launch sequence.
+// ---------------------------------------------------------------------------------
+// A structure's pass plan, kept across calls: a plan depends only on gate
+// kinds, qubits and the tile geometry, so a batch that recurs (an optimiser
+// loop, a benchmark) plans once.  The specialised kernels of its passes are
+// attached as the run-time policy compiles them (any pass size: a batch's
+// structures recur, so the second sighting pays for the compile).
+struct CachedPlan {
+    Plan plan;
+    std::vector<int> fp;     // first primitive of each gate
+    int jit_mode = -1;       // policy the kernels were attached under
+    bool ready = false;      // every step has its kernel: nothing left to attach
+    bool normalize = false;  // holds a renormalisation step (not capturable)
+};
+static std::map<std::vector<int64_t>, std::unique_ptr<CachedPlan>> g_plan_cache;
+static constexpr size_t kPlanCacheMax = 512;
+
+static int cached_plan(std::vector<int64_t> key, int n, int dtype, const lq_gate* gates, int ng, uint64_t ninst,
+                       CachedPlan** out) {
+    const int tb = default_tile_bits(n, dtype), lb = default_low_bits(n, dtype);
+    key.push_back(dtype);
+    key.push_back(tb);
+    key.push_back(lb);
+    auto it = g_plan_cache.find(key);
+    if (it == g_plan_cache.end()) {
+        auto cp = std::make_unique<CachedPlan>();
+        std::vector<Prim> base;
+        int rc = gates_to_prims(n, gates, ng, base);
+        if (rc) return rc;
+        plan_passes(n, base, tb, lb, cp->plan);
+        cp->fp = first_prims(base, ng);
+        for (const Step& st : cp->plan.steps)
+            if (st.normalize) cp->normalize = true;
+        it = g_plan_cache.emplace(std::move(key), std::move(cp)).first;
+    }
+    CachedPlan* cp = it->second.get();
+    const int mode = jit_mode();
+    if (!cp->ready || cp->jit_mode != mode) {
+        const bool all = jit_prepare_batch(cp->plan, dtype, ninst);
+        cp->ready = all && mode != LQ_JIT_OFF;
+        cp->jit_mode = mode;
+    }
+    *out = cp;
+    return LQ_OK;
+}
+
+// Circuits [first, first + count) of one structure group, laid out in the
+// batch's state buffer from state_off (amplitudes) and in the plan buffer
+// (op tables of the count instances at ops_off, the plan's tile groups at
+// groups_off, bytes).
+struct UnevenPiece {
+    CachedPlan* cp = nullptr;
+    const std::vector<int>* members = nullptr;
+    int first = 0, count = 0, n = 0;
+    uint64_t dim = 0, state_off = 0;
+    size_t nops = 0;
+    // host layout (bytes into the pinned buffer, copied as it is to the
+    // device's staging area): compact pieces hold the plan's op table once
+    // (ops_off) and 8 matrix words per op and instance (m8_off); the others
+    // hold every instance's full table at ops_off.  The plan's tile groups
+    // at groups_off.  Compact pieces' tables are expanded on the device at
+    // dev_off (bytes into the device buffer).
+    size_t ops_off = 0, m8_off = 0, groups_off = 0, dev_off = 0;
+    bool compact = false;
+    bool fill = false;  // the first pass reads the state: write |0...0> first
+};
+
+struct UnevenJob {
+    int dtype = 0, rounds = 0;
+    bool graphable = true;
+    std::vector<UnevenPiece> pieces;
+    size_t plan_bytes = 0;   // host layout
+    size_t stage_off = 0;    // device: expanded tables, then the host layout
+    uint64_t total_amps = 0;
+    BatchJob ex;  // every instance's expectation (inst_off / inst_dim)
+};
+
+// The device work of an uneven job: |0...0> only for pieces whose first
+// pass reads the state (the specialised first pass builds it in LDS), one
+// upload of every op table, then the passes round by round (pass r of every
+// piece), then one expectation launch over every instance's orbit items,
+// the per-instance sums and the result copy.
+static int uneven_launch(const UnevenJob& u, void* psi) {
+    const size_t amp = u.dtype == LQ_C128 ? 16 : 8;
+    for (const UnevenPiece& pc : u.pieces) {
+        if (!pc.fill) continue;
+        const uint64_t total = pc.dim * (uint64_t)pc.count;
+        const unsigned g = (unsigned)std::min<uint64_t>((total + 255) / 256, 65536);
+        char* base = (char*)psi + pc.state_off * amp;
+        if (u.dtype == LQ_C128)
+            hipLaunchKernelGGL(k_fill_zero_states<double2>, dim3(g), dim3(256), 0, stream(), (double2*)base, pc.dim,
+                               total);
+        else
+            hipLaunchKernelGGL(k_fill_zero_states<float2>, dim3(g), dim3(256), 0, stream(), (float2*)base, pc.dim,
+                               total);
+        LQ_LAUNCH_CHECK();
+    }
+    char* dev = (char*)g_bplan.p;
+    char* stage = dev + u.stage_off;
+    LQ_HIP_CHECK(hipMemcpyAsync(stage, g_hplan.p, u.plan_bytes, hipMemcpyHostToDevice, stream()));
+    for (const UnevenPiece& pc : u.pieces) {
+        if (!pc.compact || pc.nops == 0) continue;
+        const uint64_t words = sizeof(TileOp) / 8 * pc.nops * (uint64_t)pc.count;
+        const unsigned g = (unsigned)std::min<uint64_t>((words + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_expand_ops, dim3(g), dim3(256), 0, stream(), (uint64_t*)(dev + pc.dev_off),
+                           (const uint64_t*)(stage + pc.ops_off), (const uint64_t*)(stage + pc.m8_off),
+                           (uint64_t)pc.nops, (uint64_t)pc.count);
+        LQ_LAUNCH_CHECK();
+    }
+    for (int r = 0; r < u.rounds; ++r) {
+        for (const UnevenPiece& pc : u.pieces) {
+            const Plan& pl = pc.cp->plan;
+            if (r >= (int)pl.steps.size()) continue;
+            const Step& st = pl.steps[r];
+            char* base = (char*)psi + pc.state_off * amp;
+            int rc;
+            if (st.normalize) {
+                for (int i = 0; i < pc.count; ++i) {
+                    lq_state tmp;
+                    tmp.n = pc.n;
+                    tmp.dtype = u.dtype;
+                    tmp.dim = pc.dim;
+                    tmp.amp_bytes = amp;
+                    tmp.d_amps = base + (size_t)i * pc.dim * amp;
+                    rc = normalize_device(&tmp, nullptr);
+                    if (rc) return rc;
+                }
+                continue;
+            }
+            const TileOp* ops = (const TileOp*)(pc.compact ? dev + pc.dev_off : stage + pc.ops_off);
+            rc = launch_tile_pass(base, u.dtype, st.pass, ops, (const TileGroup*)(stage + pc.groups_off),
+                                  (uint64_t)pc.count, pc.dim, pc.nops, st.jit);
+            if (rc) return rc;
+        }
+    }
+    return expect_launch(u.ex, psi);
+}
+
+static std::vector<uint64_t> uneven_key(const UnevenJob& u, const void* psi) {
+    std::vector<uint64_t> k = {0x554eull, (uint64_t)u.dtype, (uint64_t)u.rounds, u.pieces.size(), u.plan_bytes,
+                               u.stage_off, u.total_amps, (uint64_t)psi, (uint64_t)g_bplan.p, (uint64_t)g_hplan.p};
+    for (const UnevenPiece& pc : u.pieces) {
+        for (uint64_t v : {(uint64_t)pc.count, (uint64_t)pc.n, pc.dim, pc.state_off, (uint64_t)pc.nops,
+                           (uint64_t)pc.ops_off, (uint64_t)pc.m8_off, (uint64_t)pc.groups_off, (uint64_t)pc.dev_off,
+                           (uint64_t)pc.compact, (uint64_t)pc.fill, (uint64_t)pc.cp->plan.steps.size()})
+            k.push_back(v);
+        for (const Step& st : pc.cp->plan.steps) step_key(k, st);
+    }
+    expect_key(k, u.ex);
+    return k;
+}
+
+// Prepare and run one uneven job over `pieces` (plans attached, members
+// set); energies go to energies[circuit index].
+static int uneven_run(UnevenJob& u, const lq_gate* const* circuits, const int* num_gates,
+                      const std::vector<PauliSum>& sums, double* energies, bool timing) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    const size_t amp = u.dtype == LQ_C128 ? 16 : 8;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // layout: states and plan buffer
+    size_t pb = 0, db = 0;
+    uint64_t amps = 0;
+    int ninst = 0;
+    u.rounds = 0;
+    u.graphable = true;
+    for (UnevenPiece& pc : u.pieces) {
+        const Plan& pl = pc.cp->plan;
+        pc.nops = pl.ops.size();
+        pc.state_off = amps;
+        amps += pc.dim * (uint64_t)pc.count;
+        // 4x4 ops carry 32 matrix words: those plans ship full tables
+        pc.compact = pc.count > 1;
+        for (const TileOp& t : pl.ops)
+            if (t.kind == kTQuad) pc.compact = false;
+        pc.ops_off = pb;
+        if (pc.compact) {
+            pb = al(pb + pc.nops * sizeof(TileOp));
+            pc.m8_off = pb;
+            pb = al(pb + pc.nops * (size_t)pc.count * 8 * sizeof(double));
+            pc.dev_off = db;
+            db = al(db + pc.nops * (size_t)pc.count * sizeof(TileOp));
+        } else {
+            pb = al(pb + pc.nops * (size_t)pc.count * sizeof(TileOp));
+        }
+        pc.groups_off = pb;
+        pb = al(pb + pl.groups.size() * sizeof(TileGroup));
+        pc.fill = pl.steps.empty() || pl.steps[0].normalize || !pl.steps[0].jit;
+        u.rounds = std::max(u.rounds, (int)pl.steps.size());
+        if (pc.cp->normalize) u.graphable = false;
+        ninst += pc.count;
+    }
+    u.plan_bytes = std::max<size_t>(pb, 256);
+    u.stage_off = db;
+    u.total_amps = amps;
+    int rc = g_hplan.get(u.plan_bytes);
+    if (!rc) rc = g_bplan.get(u.stage_off + u.plan_bytes);
+    if (!rc) rc = g_states.get((size_t)amps * amp);
+    if (rc) return rc;
+    // op tables (host threads over instances: each instance's matrices in
+    // its structure's plan), the plans' base tables and tile groups
+    char* hp = (char*)g_hplan.p;
+    std::vector<std::pair<int, int>> inst;  // (piece, k)
+    inst.reserve(ninst);
+    for (size_t q = 0; q < u.pieces.size(); ++q) {
+        const UnevenPiece& pc = u.pieces[q];
+        for (int k = 0; k < pc.count; ++k) inst.emplace_back((int)q, k);
+        const Plan& pl = pc.cp->plan;
+        if (pc.compact && pc.nops) std::memcpy(hp + pc.ops_off, pl.ops.data(), pc.nops * sizeof(TileOp));
+        const size_t gb = pl.groups.size() * sizeof(TileGroup);
+        if (gb) std::memcpy(hp + pc.groups_off, pl.groups.data(), gb);
+    }
+    std::vector<int> bad(ninst, 0);
+    parallel_for(ninst, [&](int i) {
+        const UnevenPiece& pc = u.pieces[inst[i].first];
+        const int c = (*pc.members)[pc.first + inst[i].second];
+        std::vector<Prim> pr;
+        if (gates_to_prims(pc.n, circuits[c], num_gates[c], pr) != LQ_OK) {
+            bad[i] = 1;
+            return;
+        }
+        const Plan& pl = pc.cp->plan;
+        if (pc.compact) {
+            double* m8 = (double*)(hp + pc.m8_off) + pc.nops * 8 * (size_t)inst[i].second;
+            for (size_t o = 0; o < pc.nops; ++o)
+                std::memcpy(m8 + 8 * o, pr[pc.cp->fp[pl.op_src[o]]].m, 8 * sizeof(double));
+        } else {
+            instance_ops(pl, pr, pc.cp->fp, (TileOp*)(hp + pc.ops_off) + pc.nops * (size_t)inst[i].second);
+        }
+    });
+    for (int i = 0; i < ninst; ++i)
+        if (bad[i]) {
+            // again on this thread, for the error message
+            const UnevenPiece& pc = u.pieces[inst[i].first];
+            std::vector<Prim> pr;
+            const int c = (*pc.members)[pc.first + inst[i].second];
+            rc = gates_to_prims(pc.n, circuits[c], num_gates[c], pr);
+            return rc ? rc : LQ_ERR_INVALID_ARGUMENT;
+        }
+    // expectation items of every instance
+    BatchJob& ex = u.ex;
+    ex = BatchJob();
+    ex.dtype = u.dtype;
+    ex.ninst = ninst;
+    ex.inst_off.resize(ninst);
+    ex.inst_dim.resize(ninst);
+    std::vector<int> obs_of(ninst);
+    for (int i = 0; i < ninst; ++i) {
+        const UnevenPiece& pc = u.pieces[inst[i].first];
+        ex.inst_off[i] = pc.state_off + pc.dim * (uint64_t)inst[i].second;
+        ex.inst_dim[i] = pc.dim;
+        obs_of[i] = (*pc.members)[pc.first + inst[i].second];
+    }
+    rc = job_expect(ex, sums, obs_of);
+    if (rc) return rc;
+    const auto t1 = clk::now();
+    rc = run_graphed(u.graphable && g_graphs_on ? uneven_key(u, g_states.p) : std::vector<uint64_t>(), u.graphable,
+                     [&] { return uneven_launch(u, g_states.p); });
+    if (rc) return rc;
+    const auto t2 = clk::now();
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
+    const auto t3 = clk::now();
+    const double* h = (const double*)g_hout.p;
+    for (int i = 0; i < ninst; ++i) energies[obs_of[i]] = (ex.blocks > 0 || !ex.lays.empty()) ? h[i] : 0.0;
+    if (timing) {
+        auto ms = [](clk::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+        std::fprintf(stderr, "lq_batch_expectations: %d circuits in %zu pieces, %d rounds: prepare %.3f ms, "
+                     "launch %.3f ms, wait %.3f ms\n", ninst, u.pieces.size(), u.rounds, ms(t1 - t0), ms(t2 - t1),
+                     ms(t3 - t2));
+    }
+    return LQ_OK;
 }
 
 }  // namespace lq
 
 using namespace lq;
 
+// Every circuit runs from |0...0> through its own gates, then its own Pauli
+// sum's expectation (PauliObservable::expectation,
+// src/optimization/observable.rs:228-231).  Circuits of one structure share a
+// pass plan (cached across calls) and run as one piece with per-instance op
+// tables; all pieces that fit the memory budget form one launch sequence
+// (captured into a hipGraph when it recurs), passes taken round by round.
 extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_gate* const* circuits,
                                      const int* num_gates, const uint8_t* const* paulis,
                                      const double* const* coeffs, const int* num_terms, int convention, int dtype,
@@ -782,72 +1153,104 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
     }
     int rc = ensure_init();
     if (rc) return rc;
-    // validate and compile observables
+    // per circuit, on host threads: gate validation, the Pauli sum, a hash
+    // of the structure; the first failing circuit is checked again on this
+    // thread for its error message
     std::vector<PauliSum> sums(count);
-    for (int c = 0; c < count; ++c) {
+    std::vector<uint64_t> shash(count);
+    auto check = [&](int c) -> int {
         if (num_qubits[c] < 1 || num_qubits[c] > 30 || num_gates[c] < 0 || (num_gates[c] > 0 && !circuits[c])) {
             set_error("circuit %d: invalid size or gate list", c);
             return LQ_ERR_INVALID_ARGUMENT;
         }
         for (int g = 0; g < num_gates[c]; ++g) {
-            rc = validate_gate(num_qubits[c], &circuits[c][g]);
-            if (rc) return rc;
+            int r = validate_gate(num_qubits[c], &circuits[c][g]);
+            if (r) return r;
         }
-        rc = compile_pauli_sum(num_qubits[c], paulis[c], coeffs[c], num_terms[c], convention, &sums[c]);
-        if (rc) return rc;
+        int r = compile_pauli_sum(num_qubits[c], paulis[c], coeffs[c], num_terms[c], convention, &sums[c]);
+        if (r) return r;
+        shash[c] = structure_hash(num_qubits[c], circuits[c], num_gates[c]);
+        return LQ_OK;
+    };
+    std::vector<int> bad(count, 0);
+    parallel_for(count, [&](int c) { bad[c] = check(c) != LQ_OK; });
+    for (int c = 0; c < count; ++c)
+        if (bad[c]) {
+            rc = check(c);
+            return rc ? rc : LQ_ERR_INVALID_ARGUMENT;
+        }
+    if (count == 0) return LQ_OK;
+    // group circuits of one structure (hash, then an exact comparison with
+    // the group's first circuit); largest states first
+    std::unordered_map<uint64_t, std::vector<int>> by_hash;
+    for (int c = 0; c < count; ++c) by_hash[shash[c]].push_back(c);
+    std::vector<std::vector<int>> groups;
+    for (auto& kv : by_hash) {
+        std::vector<int>& m = kv.second;
+        while (!m.empty()) {
+            const int r0 = m[0];
+            std::vector<int> same, rest;
+            for (int c : m)
+                (same_structure(num_qubits[r0], circuits[r0], num_gates[r0], num_qubits[c], circuits[c], num_gates[c])
+                     ? same
+                     : rest)
+                    .push_back(c);
+            groups.push_back(std::move(same));
+            m.swap(rest);
+        }
     }
-    // group circuits of one structure; largest states first
-    std::map<std::vector<int64_t>, std::vector<int>> groups;
-    for (int c = 0; c < count; ++c) groups[structure_key(num_qubits[c], circuits[c], num_gates[c])].push_back(c);
-    std::vector<std::vector<int>> order;
-    for (auto& kv : groups) order.push_back(kv.second);
-    std::sort(order.begin(), order.end(),
-              [&](const std::vector<int>& a, const std::vector<int>& b) { return num_qubits[a[0]] > num_qubits[b[0]]; });
+    std::sort(groups.begin(), groups.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
+        return num_qubits[a[0]] != num_qubits[b[0]] ? num_qubits[a[0]] > num_qubits[b[0]] : a[0] < b[0];
+    });
+    std::vector<std::vector<int64_t>> keys(groups.size());
+    std::vector<std::pair<const std::vector<int64_t>*, const std::vector<int>*>> order;
+    for (size_t k = 0; k < groups.size(); ++k) {
+        const int r0 = groups[k][0];
+        keys[k] = structure_key(num_qubits[r0], circuits[r0], num_gates[r0]);
+        order.emplace_back(&keys[k], &groups[k]);
+    }
+    if (g_plan_cache.size() + order.size() > kPlanCacheMax) g_plan_cache.clear();
     const size_t amp = dtype == LQ_C128 ? 16 : 8;
-    for (const auto& members : order) {
-        const int n = num_qubits[members[0]];
-        const int ng = num_gates[members[0]];
+    // pieces of at most `budget` bytes of states, packed into jobs of at most
+    // `budget` bytes
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
+    const size_t budget = std::max<size_t>(std::min<size_t>(freeb / 2, (size_t)96 << 30), (size_t)1 << 20);
+    std::vector<UnevenPiece> pieces;
+    for (const auto& [key, members] : order) {
+        const int n = num_qubits[(*members)[0]];
         const uint64_t dim = 1ull << n;
-        std::vector<Prim> base;
-        rc = gates_to_prims(n, circuits[members[0]], ng, base);
+        CachedPlan* cp = nullptr;
+        rc = cached_plan(*key, n, dtype, circuits[(*members)[0]], num_gates[(*members)[0]], members->size(), &cp);
         if (rc) return rc;
-        Plan plan;
-        plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
-        jit_prepare(plan, dtype, (uint64_t)members.size());
-        std::vector<int> fp = first_prims(base, ng);
-        const size_t nops = plan.ops.size();
-        const int total = (int)members.size();
-        const int chunk = chunk_instances(dim, amp, total);
-        for (int c0 = 0; c0 < total; c0 += chunk) {
-            const int cn = std::min(chunk, total - c0);
-            BatchJob job;
-            job.n = n;
-            job.dtype = dtype;
-            job.ninst = cn;
-            job.dim = dim;
-            job.plan = &plan;
-            job.nops = nops;
-            TileOp* ops = nullptr;
-            rc = job_ops(job, &ops);
-            if (rc) return rc;
-            std::vector<PauliSum> csums;
-            std::vector<int> obs_of(cn);
-            for (int k = 0; k < cn; ++k) {
-                const int c = members[c0 + k];
-                std::vector<Prim> pr;
-                rc = gates_to_prims(n, circuits[c], ng, pr);
-                if (rc) return rc;
-                instance_ops(plan, pr, fp, ops + nops * (size_t)k);
-                csums.push_back(sums[c]);
-                obs_of[k] = k;
-            }
-            std::vector<double> e(cn);
-            rc = job_expect(job, csums, obs_of);
-            if (!rc) rc = g_states.get((size_t)cn * dim * amp);
-            if (!rc) rc = job_run(job, g_states.p, e.data());
-            if (rc) return rc;
-            for (int k = 0; k < cn; ++k) energies[members[c0 + k]] = e[k];
+        const int total = (int)members->size();
+        const int per = (int)std::max<size_t>(1, std::min<size_t>(budget / (dim * amp), (size_t)total));
+        for (int c0 = 0; c0 < total; c0 += per) {
+            UnevenPiece pc;
+            pc.cp = cp;
+            pc.members = members;
+            pc.first = c0;
+            pc.count = std::min(per, total - c0);
+            pc.n = n;
+            pc.dim = dim;
+            pieces.push_back(pc);
         }
+    }
+    static const bool timing = std::getenv("LQ_BATCH_TIMING") != nullptr;
+    UnevenJob job;
+    job.dtype = dtype;
+    size_t used = 0;
+    for (size_t q = 0; q <= pieces.size(); ++q) {
+        const size_t bytes = q < pieces.size() ? pieces[q].dim * (size_t)pieces[q].count * amp : 0;
+        if (q == pieces.size() || (!job.pieces.empty() && used + bytes > budget)) {
+            rc = uneven_run(job, circuits, num_gates, sums, energies, timing);
+            if (rc) return rc;
+            job.pieces.clear();
+            used = 0;
+            if (q == pieces.size()) break;
+        }
+        job.pieces.push_back(pieces[q]);
+        used += bytes;
     }
     return LQ_OK;
 }

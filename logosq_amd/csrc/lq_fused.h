@@ -59,6 +59,11 @@ int launch_tile_pass(void* psi, int dtype, const TilePass& p, const TileOp* d_op
 // ninst: instances each pass will run over.  Does nothing inside a stream
 // capture.
 void jit_prepare(Plan& plan, int dtype, uint64_t ninst);
+// Batches of circuits that start from |0...0> (the uneven batch): as
+// jit_prepare for passes of any size, with the first pass's kernel (when
+// steps[0].jit is set) building |0...0> in LDS instead of reading the
+// state.  True when every step has a kernel.
+bool jit_prepare_batch(Plan& plan, int dtype, uint64_t ninst);
 // Compile (unless the mode is off) and load the specialised kernels a fused
 // execution of these gates will use, so that a timed run that follows does
 // not compile.

@@ -76,7 +76,10 @@ struct GroupPlan {
     std::vector<int> uphase;          // diagonal ops on outside bits only, merged into one thread phase
 };
 
-static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype) {
+// init: the pass starts from |0...0> (j_tile_zero in place of the tile
+// load), for a circuit's first pass over a state not yet written.
+static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype,
+                               bool init = false) {
     const int K = p.L + p.hb;
     const int NT = jit_threads(K);
     auto touch = [&](const TileOp& t) -> int {
@@ -200,7 +203,8 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         }
         s << "    }\n";
     }
-    s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
+    if (init) s << "    j_tile_zero<K, NT>(tile, base);\n";
+    else s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
     for (int gi = 0; gi < p.group_count; ++gi) {
         const TileGroup& G = groups[p.group_first + gi];
         const GroupPlan& P = gp[gi];
@@ -370,8 +374,10 @@ static bool jit_eligible(const TilePass& p, uint64_t ninst, bool force) {
 // Attach specialised kernels to a plan's passes (Step::jit), compiling the
 // structures the policy selects.  load=false: compile only (no device).
 // compile_now: compile every eligible structure at once, whatever the mode
-// (explicit requests); any_size: ignore the minimum pass size.
-static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bool compile_now, bool any_size) {
+// (explicit requests); any_size: ignore the minimum pass size; init_first:
+// the first pass builds |0...0> in LDS instead of reading the state.
+static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bool compile_now, bool any_size,
+                            bool init_first = false) {
     for (Step& s : plan.steps) s.jit = nullptr;
     if (g_jit_mode == LQ_JIT_OFF && !compile_now) return 0;
     const bool always = compile_now || g_jit_mode == LQ_JIT_ALWAYS;
@@ -379,7 +385,8 @@ static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bo
     std::vector<std::pair<const std::string*, JitKernel*>> todo;
     for (Step& s : plan.steps) {
         if (s.normalize || !jit_eligible(s.pass, ninst, any_size || g_jit_mode == LQ_JIT_ALWAYS)) continue;
-        std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype);
+        std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype,
+                                      init_first && &s == &plan.steps[0]);
         auto it = g_jit.find(src);
         if (it == g_jit.end()) {
             if (g_jit.size() >= kJitMaxStructures) continue;  // bounded cache: new structures stay generic
@@ -412,6 +419,16 @@ void jit_prepare(Plan& plan, int dtype, uint64_t ninst) {
         return;
     }
     (void)jit_prepare_impl(plan, dtype, ninst, true, false, false);
+}
+
+bool jit_prepare_batch(Plan& plan, int dtype, uint64_t ninst) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream(), &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        for (Step& s : plan.steps) s.jit = nullptr;
+        return false;
+    }
+    const int ready = jit_prepare_impl(plan, dtype, ninst, true, false, true, true);
+    return ready == (int)plan.steps.size();
 }
 
 int jit_warm(int n, int dtype, const lq_gate* gates, int count) {

@@ -377,6 +377,24 @@ __device__ __forceinline__ void j_tile_load(V* tile, const V* __restrict__ s, ui
     __syncthreads();
 }
 
+// The tile of |0...0> instead of a load: element 0 of the tile whose base
+// is index 0 is 1, everything else 0 (a circuit's first pass, when the
+// state has not been written yet).
+template <int K, int NT, typename V>
+__device__ __forceinline__ void j_tile_zero(V* tile, uint64_t base) {
+    constexpr int EPT = (1 << K) / NT;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const uint32_t e = tid + (uint32_t)j * NT;
+        V v;
+        v.x = (base == 0 && e == 0) ? 1 : 0;
+        v.y = 0;
+        tile[swz(e)] = v;
+    }
+    __syncthreads();
+}
+
 template <int K, int HB, int NT, typename V>
 __device__ __forceinline__ void j_tile_store(const V* tile, V* __restrict__ s, uint64_t base, const uint64_t* hoff) {
     constexpr int EPT = (1 << K) / NT;

@@ -65,6 +65,47 @@ def test_uneven_batch_matches_oracle(hip, dtype, tol):
         assert abs(e - want) <= tol, (n, e, want)
 
 
+@pytest.mark.parametrize("dtype,tol", [(lq.C128, 1e-10), (lq.C64, 1e-5)])
+def test_uneven_batch_one_sequence_every_call(hip, dtype, tol):
+    """One uneven batch through every stage of its launch sequence, each call
+    against the oracle: circuits with no gates (|0...0> written by a fill),
+    fewer than 8 qubits (generic pass kernel, fill), a renormalising
+    MatrixGate (per-instance normalize step, not captured), and structures
+    that recur, whose first pass is specialised to build |0...0> in LDS once
+    compiled (second call) and whose sequence is captured (third) and
+    replayed (fourth)."""
+    from tests.test_oracle import G
+    say("uneven batch, every call, dtype=%d" % dtype)
+    rng = np.random.default_rng(23)
+    items = list(algorithms.uneven_batch_workload(count=20, nmin=6, nmax=15, layers=2, terms=6, seed=29))
+    for n in (9, 12):
+        items.append((n, [], rng.integers(0, 4, (5, n)).astype(np.uint8), rng.normal(size=5)))
+    m = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0] * 1.2
+    dense = [G(abi.RY, [q], [0.3 + q]) for q in range(10)] + [G(abi.MATRIX1, [4], (), m.reshape(-1),
+                                                                  abi.GATE_FLAG_RENORMALIZE)]
+    dense += [G(abi.CNOT, [q, q + 1]) for q in range(9)]
+    with_dense = items + [(10, dense, rng.integers(0, 4, (7, 10)).astype(np.uint8), rng.normal(size=7))]
+    want = []
+    for n, gates, codes, coeffs in with_dense:
+        psi = oracle.execute(oracle.zero_state(n), n, gates)
+        want.append(oracle.pauli_sum(psi, n, codes, coeffs))
+    want = np.array(want)
+    results = []
+    for call in range(4):
+        got = lq.batch_expectations(items, dtype)
+        err = np.abs(got - want[:len(items)])
+        say("  call %d: max |dE| %.3e" % (call, err.max()))
+        assert err.max() <= tol, (call, int(np.argmax(err)))
+        results.append(got)
+    # eager, captured and replayed runs of the same kernels agree bit for bit
+    assert np.array_equal(results[1], results[2]) and np.array_equal(results[2], results[3])
+    for call in range(2):
+        got = lq.batch_expectations(with_dense, dtype)
+        err = np.abs(got - want)
+        say("  with a MatrixGate, call %d: max |dE| %.3e" % (call, err.max()))
+        assert err.max() <= tol, (call, int(np.argmax(err)))
+
+
 @pytest.mark.parametrize("n,layers", [(3, 2), (8, 3), (14, 2)])
 def test_adjoint_gradient_matches_param_shift(hip, n, layers):
     """The adjoint gradient equals the oracle's parameter-shift gradient
