@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -47,6 +48,7 @@ struct JitKernel {
 
 static int g_jit_mode = LQ_JIT_REPEAT;
 static constexpr uint64_t kJitMinTiles = 1024;
+static constexpr uint64_t kStreamBytes = 1ull << 30;
 static constexpr size_t kJitMaxStructures = 4096;
 static std::unordered_map<std::string, std::unique_ptr<JitKernel>> g_jit;
 static long long g_jit_compiled = 0, g_jit_failed = 0, g_jit_launches = 0;
@@ -77,9 +79,10 @@ struct GroupPlan {
 };
 
 // init: the pass starts from |0...0> (j_tile_zero in place of the tile
-// load), for a circuit's first pass over a state not yet written.
+// load), for a circuit's first pass over a state not yet written.  stream:
+// nontemporal global loads and stores (states far beyond the caches).
 static std::string pass_source(const TilePass& p, const TileOp* ops, const TileGroup* groups, int dtype,
-                               bool init = false) {
+                               bool init = false, bool stream = false) {
     const int K = p.L + p.hb;
     const int NT = jit_threads(K);
     auto touch = [&](const TileOp& t) -> int {
@@ -203,8 +206,9 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         }
         s << "    }\n";
     }
+    const char* sm = stream ? "true" : "false";
     if (init) s << "    j_tile_zero<K, NT>(tile, base);\n";
-    else s << "    j_tile_load<K, HB, NT>(tile, st, base, hoff);\n";
+    else s << "    j_tile_load<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
     for (int gi = 0; gi < p.group_count; ++gi) {
         const TileGroup& G = groups[p.group_first + gi];
         const GroupPlan& P = gp[gi];
@@ -272,7 +276,7 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         }
         s << "    }, " << flips(1) << ", " << flips(2) << ");\n";
     }
-    s << "    j_tile_store<K, HB, NT>(tile, st, base, hoff);\n";
+    s << "    j_tile_store<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
     s << "  }\n}\n";
     return s.str();
 }
@@ -381,12 +385,16 @@ static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bo
     for (Step& s : plan.steps) s.jit = nullptr;
     if (g_jit_mode == LQ_JIT_OFF && !compile_now) return 0;
     const bool always = compile_now || g_jit_mode == LQ_JIT_ALWAYS;
+    // the pass's states against the caches (256 MB Infinity Cache): passes
+    // over more than kStreamBytes stream through nontemporally
+    const double bytes = (double)ninst * std::ldexp(1.0, plan.n) * (dtype == LQ_C128 ? 16.0 : 8.0);
+    const bool stream = bytes > (double)kStreamBytes;
     std::vector<std::pair<Step*, JitKernel*>> use;
     std::vector<std::pair<const std::string*, JitKernel*>> todo;
     for (Step& s : plan.steps) {
         if (s.normalize || !jit_eligible(s.pass, ninst, any_size || g_jit_mode == LQ_JIT_ALWAYS)) continue;
         std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype,
-                                      init_first && &s == &plan.steps[0]);
+                                      init_first && &s == &plan.steps[0], stream);
         auto it = g_jit.find(src);
         if (it == g_jit.end()) {
             if (g_jit.size() >= kJitMaxStructures) continue;  // bounded cache: new structures stay generic

@@ -365,13 +365,19 @@ __device__ __forceinline__ uint64_t j_gidx(uint64_t base, const uint64_t* hoff, 
     return base | (e & ((1u << L) - 1u)) | hoff[e >> L];
 }
 
-template <int K, int HB, int NT, typename V>
+// STREAM: nontemporal global accesses, for passes whose states far exceed
+// the caches (a value read once per pass gains nothing from being kept).
+template <int K, int HB, int NT, bool STREAM = false, typename V>
 __device__ __forceinline__ void j_tile_load(V* tile, const V* __restrict__ s, uint64_t base, const uint64_t* hoff) {
     constexpr int EPT = (1 << K) / NT;
     const uint32_t tid = threadIdx.x;
     V r[EPT];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) r[j] = s[j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT)];
+    for (int j = 0; j < EPT; ++j) {
+        const V* a = s + j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT);
+        if constexpr (STREAM) r[j] = __builtin_nontemporal_load(a);
+        else r[j] = *a;
+    }
 #pragma unroll
     for (int j = 0; j < EPT; ++j) tile[swz(tid + (uint32_t)j * NT)] = r[j];
     __syncthreads();
@@ -395,7 +401,7 @@ __device__ __forceinline__ void j_tile_zero(V* tile, uint64_t base) {
     __syncthreads();
 }
 
-template <int K, int HB, int NT, typename V>
+template <int K, int HB, int NT, bool STREAM = false, typename V>
 __device__ __forceinline__ void j_tile_store(const V* tile, V* __restrict__ s, uint64_t base, const uint64_t* hoff) {
     constexpr int EPT = (1 << K) / NT;
     const uint32_t tid = threadIdx.x;
@@ -403,7 +409,11 @@ __device__ __forceinline__ void j_tile_store(const V* tile, V* __restrict__ s, u
 #pragma unroll
     for (int j = 0; j < EPT; ++j) r[j] = tile[swz(tid + (uint32_t)j * NT)];
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) s[j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT)] = r[j];
+    for (int j = 0; j < EPT; ++j) {
+        V* a = s + j_gidx<K, HB>(base, hoff, tid + (uint32_t)j * NT);
+        if constexpr (STREAM) __builtin_nontemporal_store(r[j], a);
+        else *a = r[j];
+    }
     __syncthreads();
 }
 
