@@ -49,6 +49,10 @@ struct JitKernel {
 static int g_jit_mode = LQ_JIT_REPEAT;
 static constexpr uint64_t kJitMinTiles = 1024;
 static constexpr uint64_t kStreamBytes = 1ull << 30;
+// First / last groups of a pass straight from / to the state (on by default;
+// LQ_JIT_DIRECT=0 in the environment keeps every pass on LDS tile loads and
+// stores, for comparison)
+static const bool direct_groups = !(std::getenv("LQ_JIT_DIRECT") && std::getenv("LQ_JIT_DIRECT")[0] == '0');
 static constexpr size_t kJitMaxStructures = 4096;
 static std::unordered_map<std::string, std::unique_ptr<JitKernel>> g_jit;
 static long long g_jit_compiled = 0, g_jit_failed = 0, g_jit_launches = 0;
@@ -164,6 +168,47 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         }
     }
 
+    // The first (last) group reads (writes) the state directly instead of
+    // through a tile load (store) in LDS when its subcube bits leave the
+    // lowest bits of the tile to the lanes, so that a wave's access still
+    // covers whole 128-byte lines; a pass of one such group touches no LDS.
+    const int lmin = dtype == LQ_C128 ? 3 : 4;
+    auto lanes_low = [&](int gi) {
+        const TileGroup& G = groups[p.group_first + gi];
+        int m = K;
+        for (int k = 0; k < kGroupBits; ++k) m = std::min(m, (int)G.spos[k]);
+        return p.L >= lmin && m >= lmin;
+    };
+    const bool dfirst = direct_groups && p.group_count >= 1 && lanes_low(0);
+    const bool single = dfirst && p.group_count == 1;
+    const bool dlast = direct_groups && p.group_count >= 2 && lanes_low(p.group_count - 1);
+    // a direct first group that reads the state is issued one tile ahead
+    // when the registers allow: blocks of 256 threads run two waves per SIMD
+    // (256 VGPRs each); at 512 threads (K = 13) the extra 16 amplitudes per
+    // thread cost more than the overlap gains (complex64 brickwork 127 ->
+    // 138 ms)
+    const bool pipe = dfirst && !init && NT <= 256;
+    // gather (which = 1) / scatter (2) slot flips of group gi, as a lambda's text
+    auto flips_of = [&](int gi, int which) {
+        const TileGroup& G = groups[p.group_first + gi];
+        const GroupPlan& P = gp[gi];
+        const int o0 = G.op_first, o1 = G.op_first + G.op_count;
+        std::ostringstream f;
+        f << "[&](uint64_t gb) -> uint32_t { uint32_t f = 0u;";
+        for (int o = o0; o < o1; ++o) {
+            if (P.fold[o - o0] != which) continue;
+            const TileOp& t = ops[p.op_first + o];
+            f << " if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) f ^= "
+              << (1u << G.spos[t.sp0]) << "u;";
+        }
+        f << " return f; }";
+        return f.str();
+    };
+    auto pos_of = [&](int gi) {
+        const TileGroup& G = groups[p.group_first + gi];
+        return std::to_string(G.spos[0]) + ", " + std::to_string(G.spos[1]) + ", " + std::to_string(G.spos[2]) +
+               ", " + std::to_string(G.spos[3]);
+    };
     std::ostringstream s;
     s << "#include \"lq_tile_ops.h\"\nusing namespace lq;\n";
     if (dtype == LQ_C128)
@@ -183,6 +228,14 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;\n";
     s << "  const uint64_t w0 = (uint64_t)blockIdx.x * per, w1 = w0 + per < total ? w0 + per : total;\n";
     if (!runs.empty()) s << "  uint32_t tab_inst = 0xffffffffu;\n";
+    const char* sm = stream ? "true" : "false";
+    if (pipe) {
+        s << "  V pre[16];\n  const auto gf0 = " << flips_of(0, 1) << ";\n";
+        s << "  if (w0 < w1) {\n    const uint32_t in0 = __builtin_amdgcn_readfirstlane((uint32_t)(w0 / p.ntiles));\n";
+        s << "    j_first_fetch<K, HB, NT, " << pos_of(0) << ", " << sm
+          << ">(pre, psi + (uint64_t)in0 * inst_stride, j_tile_base<K, HB>(w0 - (uint64_t)in0 * p.ntiles, p), hoff, p, "
+             "gf0);\n  }\n";
+    }
     s << "  for (uint64_t w = w0; w < w1; ++w) {\n";
     s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));\n";
     s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
@@ -193,8 +246,12 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         // when the block reaches a new instance (its tiles are contiguous,
         // so mostly once); the tile load's barrier publishes them, and the
         // previous tile's last barrier has retired every read of the old ones
-        s << "    if (inst != tab_inst) {\n      tab_inst = inst;\n"
-             "      const int ts = (int)(threadIdx.x & 15u), tg = (int)(threadIdx.x >> 4);\n";
+        s << "    if (inst != tab_inst) {\n      tab_inst = inst;\n";
+        // with a first group read straight from the state, no tile load's
+        // barrier follows: barriers of their own retire the previous tile's
+        // reads of the old tables and publish the new ones
+        if (dfirst) s << "      __syncthreads();\n";
+        s << "      const int ts = (int)(threadIdx.x & 15u), tg = (int)(threadIdx.x >> 4);\n";
         for (size_t r = 0; r < runs.size(); ++r) {
             s << "      if (tg == " << (int)(r % (size_t)(NT / 16)) << ") {\n        double pr = 1.0, pi = 0.0;\n";
             for (int o : runs[r]) {
@@ -204,29 +261,30 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             }
             s << "        ptab[" << r * 16 << " + ts] = V{(T)pr, (T)pi};\n      }\n";
         }
+        if (dfirst) s << "      __syncthreads();\n";
         s << "    }\n";
     }
-    const char* sm = stream ? "true" : "false";
-    if (init) s << "    j_tile_zero<K, NT>(tile, base);\n";
-    else s << "    j_tile_load<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
+    if (!dfirst) {
+        if (init) s << "    j_tile_zero<K, NT>(tile, base);\n";
+        else s << "    j_tile_load<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
+    }
     for (int gi = 0; gi < p.group_count; ++gi) {
         const TileGroup& G = groups[p.group_first + gi];
         const GroupPlan& P = gp[gi];
-        const int o0 = G.op_first, o1 = G.op_first + G.op_count;
-        auto flips = [&](int which) {
-            std::ostringstream f;
-            f << "[&](uint64_t gb) -> uint32_t { uint32_t f = 0u;";
-            for (int o = o0; o < o1; ++o) {
-                if (P.fold[o - o0] != which) continue;
-                const TileOp& t = ops[p.op_first + o];
-                f << " if ((gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout << std::dec << "ull) f ^= "
-                  << (1u << G.spos[t.sp0]) << "u;";
-            }
-            f << " return f; }";
-            return f.str();
-        };
-        s << "    j_group<K, HB, NT, " << G.spos[0] << ", " << G.spos[1] << ", " << G.spos[2] << ", " << G.spos[3]
-          << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
+        auto flips = [&](int which) { return flips_of(gi, which); };
+        const std::string pos = std::to_string(G.spos[0]) + ", " + std::to_string(G.spos[1]) + ", " +
+                                std::to_string(G.spos[2]) + ", " + std::to_string(G.spos[3]);
+        if (gi == 0 && pipe)
+            s << "    j_first_apply<K, HB, NT, " << pos << ", " << sm << ", " << (single ? "true" : "false")
+              << ", T>(tile, st, pre, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
+        else if (gi == 0 && dfirst)
+            s << "    j_group_first<K, HB, NT, " << pos << ", " << sm << ", " << (init ? "true" : "false") << ", "
+              << (single ? "true" : "false") << ", T>(tile, st, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
+        else if (gi == p.group_count - 1 && dlast)
+            s << "    j_group_last<K, HB, NT, " << pos << ", " << sm << ", " << (dfirst ? "false" : "true")
+              << ", T>(tile, st, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
+        else
+            s << "    j_group<K, HB, NT, " << pos << ", T>(tile, base, hoff, [&](T* xr, T* xi, uint64_t gb) {\n";
         auto gbit = [](int b) { return "(int)((gb >> " + std::to_string(b) + ") & 1ull)"; };
         for (const auto& [what, idx] : P.items) {
             s << "      ";
@@ -274,9 +332,19 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
                     break;
             }
         }
-        s << "    }, " << flips(1) << ", " << flips(2) << ");\n";
+        if (gi == 0 && pipe) {
+            s << "    }, " << flips(2) << ");\n";
+            // the next tile's first gather, in flight while this tile's other groups run
+            s << "    if (w + 1 < w1) {\n      const uint32_t in1 = __builtin_amdgcn_readfirstlane((uint32_t)((w + 1) / "
+                 "p.ntiles));\n";
+            s << "      j_first_fetch<K, HB, NT, " << pos << ", " << sm
+              << ">(pre, psi + (uint64_t)in1 * inst_stride, j_tile_base<K, HB>(w + 1 - (uint64_t)in1 * p.ntiles, p), "
+                 "hoff, p, gf0);\n    }\n";
+        } else {
+            s << "    }, " << flips(1) << ", " << flips(2) << ");\n";
+        }
     }
-    s << "    j_tile_store<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
+    if (!dlast && !single) s << "    j_tile_store<K, HB, NT, " << sm << ">(tile, st, base, hoff);\n";
     s << "  }\n}\n";
     return s.str();
 }

@@ -469,4 +469,215 @@ __device__ __forceinline__ void j_group(V* tile, uint64_t base, const uint64_t* 
     __syncthreads();
 }
 
+// Global offset of subcube slot c relative to slot 0: the tile's element ->
+// index map is a bit spread (linear over XOR), so slot c of a subcube at
+// global index g sits at g ^ d(c), d(c) the XOR of the spread subcube bits.
+template <int K, int HB, int S>
+__device__ __forceinline__ uint64_t j_spread_bit(const TilePass& p) {
+    constexpr int L = K - HB;
+    if constexpr (S < L) return 1ull << S;
+    else return 1ull << p.hpos[S - L];
+}
+
+// The first group of a tile, straight from the state: each thread gathers
+// its subcube's 16 amplitudes from global memory (INIT: from |0...0>, the
+// state not yet written), applies the group, and, unless LAST, scatters into
+// the LDS tile after a barrier that retires the previous tile's LDS reads.
+// LAST (a pass of one group): the subcube goes straight back to the state
+// and the pass touches no LDS.  Coalesced when the subcube bits leave the
+// lowest tile bits to the lanes (whole 128-byte lines per wave access).
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, bool INIT, bool LAST, typename T,
+          typename V, typename F, typename GF, typename SF>
+__device__ __forceinline__ void j_group_first(V* tile, V* __restrict__ st, uint64_t base, const uint64_t* hoff,
+                                              const TilePass& p, F&& body, GF&& gflip, SF&& sflip) {
+    constexpr uint32_t NSUB = (1u << K) >> 4;
+    static_assert(NSUB <= (uint32_t)NT, "one subcube per thread at most");
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const bool act = NSUB == (uint32_t)NT || tid < NSUB;
+    const uint64_t d0 = j_spread_bit<K, HB, S0>(p), d1 = j_spread_bit<K, HB, S1>(p);
+    const uint64_t d2 = j_spread_bit<K, HB, S2>(p), d3 = j_spread_bit<K, HB, S3>(p);
+    T xr[16], xi[16];
+    uint32_t eb = 0;
+    uint64_t gb = 0;
+    if (act) {
+        eb = ins0(ins0(ins0(ins0(tid, S0), S1), S2), S3);
+        gb = j_gidx<K, HB>(base, hoff, eb);
+        const uint64_t g0 = j_gidx<K, HB>(base, hoff, eb ^ gflip(gb));
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint64_t gi = g0 ^ ((c & 1) ? d0 : 0ull) ^ ((c & 2) ? d1 : 0ull) ^ ((c & 4) ? d2 : 0ull) ^
+                                ((c & 8) ? d3 : 0ull);
+            V v;
+            if constexpr (INIT) {
+                v.x = gi == 0 ? 1 : 0;
+                v.y = 0;
+            } else if constexpr (STREAM) {
+                v = __builtin_nontemporal_load(st + gi);
+            } else {
+                v = st[gi];
+            }
+            xr[c] = v.x;
+            xi[c] = v.y;
+        }
+        body(xr, xi, gb);
+    }
+    if constexpr (LAST) {
+        if (act) {
+            const uint64_t g0 = j_gidx<K, HB>(base, hoff, eb ^ sflip(gb));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const uint64_t gi = g0 ^ ((c & 1) ? d0 : 0ull) ^ ((c & 2) ? d1 : 0ull) ^ ((c & 4) ? d2 : 0ull) ^
+                                    ((c & 8) ? d3 : 0ull);
+                V v;
+                v.x = xr[c];
+                v.y = xi[c];
+                if constexpr (STREAM) __builtin_nontemporal_store(v, st + gi);
+                else st[gi] = v;
+            }
+        }
+    } else {
+        __syncthreads();
+        if (act) {
+            const uint32_t ss = swz(eb) ^ swz(sflip(gb));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const uint32_t oc = ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                                    ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+                V v;
+                v.x = xr[c];
+                v.y = xi[c];
+                tile[ss ^ swz(oc)] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The first group split in two so that a tile's gather can be issued one
+// tile ahead (software pipelining across the tiles of a block):
+// j_first_fetch loads the subcube of the tile at `base` into r[16] without
+// waiting for it; j_first_apply applies the group to r and scatters it (into
+// the LDS tile after a barrier, or, LAST, straight back to the state).
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, typename V, typename GF>
+__device__ __forceinline__ void j_first_fetch(V* r, const V* __restrict__ st, uint64_t base, const uint64_t* hoff,
+                                              const TilePass& p, GF&& gflip) {
+    constexpr uint32_t NSUB = (1u << K) >> 4;
+    static_assert(NSUB <= (uint32_t)NT, "one subcube per thread at most");
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (NSUB == (uint32_t)NT || tid < NSUB) {
+        const uint64_t d0 = j_spread_bit<K, HB, S0>(p), d1 = j_spread_bit<K, HB, S1>(p);
+        const uint64_t d2 = j_spread_bit<K, HB, S2>(p), d3 = j_spread_bit<K, HB, S3>(p);
+        const uint32_t eb = ins0(ins0(ins0(ins0(tid, S0), S1), S2), S3);
+        const uint64_t gb = j_gidx<K, HB>(base, hoff, eb);
+        const uint64_t g0 = j_gidx<K, HB>(base, hoff, eb ^ gflip(gb));
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint64_t gi = g0 ^ ((c & 1) ? d0 : 0ull) ^ ((c & 2) ? d1 : 0ull) ^ ((c & 4) ? d2 : 0ull) ^
+                                ((c & 8) ? d3 : 0ull);
+            if constexpr (STREAM) r[c] = __builtin_nontemporal_load(st + gi);
+            else r[c] = st[gi];
+        }
+    }
+}
+
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, bool LAST, typename T, typename V,
+          typename F, typename SF>
+__device__ __forceinline__ void j_first_apply(V* tile, V* __restrict__ st, const V* r, uint64_t base,
+                                              const uint64_t* hoff, const TilePass& p, F&& body, SF&& sflip) {
+    constexpr uint32_t NSUB = (1u << K) >> 4;
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const bool act = NSUB == (uint32_t)NT || tid < NSUB;
+    T xr[16], xi[16];
+    uint32_t eb = 0;
+    uint64_t gb = 0;
+    if (act) {
+        eb = ins0(ins0(ins0(ins0(tid, S0), S1), S2), S3);
+        gb = j_gidx<K, HB>(base, hoff, eb);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            xr[c] = r[c].x;
+            xi[c] = r[c].y;
+        }
+        body(xr, xi, gb);
+    }
+    if constexpr (LAST) {
+        if (act) {
+            const uint64_t d0 = j_spread_bit<K, HB, S0>(p), d1 = j_spread_bit<K, HB, S1>(p);
+            const uint64_t d2 = j_spread_bit<K, HB, S2>(p), d3 = j_spread_bit<K, HB, S3>(p);
+            const uint64_t g0 = j_gidx<K, HB>(base, hoff, eb ^ sflip(gb));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const uint64_t gi = g0 ^ ((c & 1) ? d0 : 0ull) ^ ((c & 2) ? d1 : 0ull) ^ ((c & 4) ? d2 : 0ull) ^
+                                    ((c & 8) ? d3 : 0ull);
+                V v;
+                v.x = xr[c];
+                v.y = xi[c];
+                if constexpr (STREAM) __builtin_nontemporal_store(v, st + gi);
+                else st[gi] = v;
+            }
+        }
+    } else {
+        __syncthreads();
+        if (act) {
+            const uint32_t ss = swz(eb) ^ swz(sflip(gb));
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const uint32_t oc = ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                                    ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+                V v;
+                v.x = xr[c];
+                v.y = xi[c];
+                tile[ss ^ swz(oc)] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The last group of a tile, straight to the state: gathers from the LDS tile,
+// applies the group and stores the subcube to global memory.  BAR: a closing
+// barrier (needed when the next tile's first phase writes the LDS tile
+// without one of its own).
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, bool BAR, typename T, typename V,
+          typename F, typename GF, typename SF>
+__device__ __forceinline__ void j_group_last(const V* tile, V* __restrict__ st, uint64_t base, const uint64_t* hoff,
+                                             const TilePass& p, F&& body, GF&& gflip, SF&& sflip) {
+    constexpr uint32_t NSUB = (1u << K) >> 4;
+    static_assert(NSUB <= (uint32_t)NT, "one subcube per thread at most");
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (NSUB == (uint32_t)NT || tid < NSUB) {
+        const uint64_t d0 = j_spread_bit<K, HB, S0>(p), d1 = j_spread_bit<K, HB, S1>(p);
+        const uint64_t d2 = j_spread_bit<K, HB, S2>(p), d3 = j_spread_bit<K, HB, S3>(p);
+        const uint32_t eb = ins0(ins0(ins0(ins0(tid, S0), S1), S2), S3);
+        const uint64_t gb = j_gidx<K, HB>(base, hoff, eb);
+        const uint32_t sg = swz(eb) ^ swz(gflip(gb));
+        T xr[16], xi[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint32_t oc = ((uint32_t)(c & 1) << S0) | ((uint32_t)((c >> 1) & 1) << S1) |
+                                ((uint32_t)((c >> 2) & 1) << S2) | ((uint32_t)((c >> 3) & 1) << S3);
+            const V v = tile[sg ^ swz(oc)];
+            xr[c] = v.x;
+            xi[c] = v.y;
+        }
+        body(xr, xi, gb);
+        const uint64_t g0 = j_gidx<K, HB>(base, hoff, eb ^ sflip(gb));
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint64_t gi = g0 ^ ((c & 1) ? d0 : 0ull) ^ ((c & 2) ? d1 : 0ull) ^ ((c & 4) ? d2 : 0ull) ^
+                                ((c & 8) ? d3 : 0ull);
+            V v;
+            v.x = xr[c];
+            v.y = xi[c];
+            if constexpr (STREAM) __builtin_nontemporal_store(v, st + gi);
+            else st[gi] = v;
+        }
+    }
+    if constexpr (BAR) __syncthreads();
+}
+
 }  // namespace lq
