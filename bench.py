@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the gradient / uneven-batch workloads")
+    ap.add_argument("--workload", choices=["brickwork", "uneven", "gradient"], default="brickwork",
+                    help="brickwork: the headline run; uneven / gradient: only that workload (one JSON line)")
     return ap.parse_args()
 
 
@@ -248,6 +250,31 @@ def uneven_batch_bench(lq, reps=3):
                      "state_GBps": round(amps * c / best / 1e9, 1), "first_call_s": round(t_first, 4)}
     d = np.abs(energies["c64"] - energies["c128"])
     res["c64_vs_c128_abs_err"] = {"max": float(d.max()), "mean": float(d.mean())}
+    # state traffic the batch implies, counted as if every access went to
+    # HBM (small states are served from the caches): per circuit, its fused
+    # plan's passes read and write the state once each (the first pass only
+    # writes: it builds |0...0> in LDS), and its Pauli sum reads it once per
+    # orbit item (up to four linearly independent flip masks per read)
+    by_n = {}
+    for n, gates, codes, coeffs in items:
+        if n not in by_n:
+            by_n[n] = {dt: num_passes(lq, n, dt, gates) for dt in (0, 1)}
+    reads = []
+    for n, gates, codes, coeffs in items:
+        masks = set()
+        for row in codes:
+            x = 0
+            for q, p in enumerate(row):
+                if p in (1, 2):
+                    x |= 1 << q
+            if x:
+                masks.add(x)
+        reads.append(max(1, -(-len(masks) // 4)))
+    for name, dtype in (("c64", 1), ("c128", 0)):
+        c = 16 if dtype == 0 else 8
+        traffic = sum(((2 * by_n[it[0]][dtype] - 1) + r) * (1 << it[0]) * c for it, r in zip(items, reads))
+        res[name]["modelled_state_traffic_GB"] = round(traffic / 1e9, 2)
+        res[name]["modelled_state_traffic_GBps"] = round(traffic / res[name]["s_per_batch"] / 1e9, 1)
     return res
 
 
@@ -604,6 +631,12 @@ def main():
     import logosq_amd as lq
     from logosq_amd import abi
     lq.load()
+    if a.workload != "brickwork":
+        # one side workload alone (for profiling it)
+        run = {"uneven": lambda: uneven_batch_bench(lq),
+               "gradient": lambda: ps_gradient_bench(lq, cpu=not a.no_cpu)}[a.workload]
+        print(json.dumps({"workload": a.workload, "result": run()}))
+        return
     dtype = lq.C128 if a.dtype == "c128" else lq.C64
     c = 16 if dtype == lq.C128 else 8
     mode = abi.EXEC_PER_GATE if a.mode == "per_gate" else abi.EXEC_FUSED
