@@ -180,7 +180,11 @@ def ps_gradient_bench(lq, reps=3, cpu=True):
         g = np.zeros(P)
         e = ctypes.c_double()
         ts = []
-        for _ in range(reps if mode == abi.GRAD_BATCHED else 1):
+        # batched: the first call runs the generic pass kernel, the second
+        # compiles the specialised ones, the third captures the launch
+        # sequence; the steady state is the fastest of `reps` calls after that
+        calls = reps + 3 if mode == abi.GRAD_BATCHED else 1
+        for _ in range(calls):
             t0 = time.perf_counter()
             lq._lib.call("lq_param_shift_gradient", 16, dtype, arr, len(tmpl),
                          params.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), P, math.pi / 2,
@@ -188,12 +192,15 @@ def ps_gradient_bench(lq, reps=3, cpu=True):
                          coeffs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(coeffs), 0, mode,
                          ctypes.byref(e), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
             ts.append(time.perf_counter() - t0)
-        out[name] = {"s_per_gradient": min(ts), "energy": e.value, "grad": g.copy()}
+        out[name] = {"s_per_gradient": min(ts[-reps:]) if mode == abi.GRAD_BATCHED else ts[0],
+                     "first_call_s": ts[0], "energy": e.value, "grad": g.copy()}
     ref = out["sequential_c128"]["grad"]
     res = {"workload": "PS gradient, 16 qubits, L=6 RY/RZ+CZ ring, P=192, XYZ ring (48 terms), 385 circuits"}
     for k, v in out.items():
         res[k] = {"ms_per_gradient": round(v["s_per_gradient"] * 1e3, 3),
                   "max_abs_grad_diff_vs_sequential_c128": float(np.max(np.abs(v["grad"] - ref)))}
+        if k.startswith("batched"):
+            res[k]["first_call_ms"] = round(v["first_call_s"] * 1e3, 3)
     res["speedup_batched_vs_sequential"] = round(out["sequential_c128"]["s_per_gradient"] /
                                                  out["batched_c128"]["s_per_gradient"], 2)
     if cpu:
