@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -30,7 +31,8 @@ std::recursive_mutex& api_mutex() {
 // ---- library state ----------------------------------------------------------------
 static std::mutex g_mu;
 static bool g_inited = false;
-static hipStream_t g_stream = nullptr;
+// Written once under g_mu; read lock-free by stream() on every launch.
+static std::atomic<hipStream_t> g_stream{nullptr};
 static void* g_scratch = nullptr;
 static size_t g_scratch_bytes = 0;
 static void* g_scratch2 = nullptr;
@@ -48,17 +50,27 @@ int ensure_init() {
         return LQ_ERR_DEVICE;
     }
     // Default-priority, non-blocking stream: all library work is ordered on it.
-    LQ_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    hipStream_t s = nullptr;
+    LQ_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    g_stream.store(s, std::memory_order_release);
     g_inited = true;
     return LQ_OK;
 }
 
-hipStream_t stream() { return g_stream; }
+// The library stream, created at the first use by any entry point (work put
+// on the legacy null stream instead would run, but could not be captured
+// into a graph).  nullptr only when no device can be initialised.
+hipStream_t stream() {
+    hipStream_t s = g_stream.load(std::memory_order_acquire);
+    if (s) return s;
+    (void)ensure_init();
+    return g_stream.load(std::memory_order_acquire);
+}
 
 static void* grow(void*& buf, size_t& have, size_t bytes) {
     if (bytes <= have) return buf;
     if (buf) {
-        if (hipStreamSynchronize(g_stream) != hipSuccess) return nullptr;
+        if (hipStreamSynchronize(stream()) != hipSuccess) return nullptr;
         (void)hipFree(buf);
         buf = nullptr;
         have = 0;
@@ -272,13 +284,13 @@ int reduce_norm_sqr(const lq_state* st, double* d_out) {
         return LQ_ERR_OUT_OF_MEMORY;
     }
     if (st->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_sumsq<double2>, dim3(nb), dim3(kRedThreads), 0, g_stream, (const double2*)st->d_amps,
+        hipLaunchKernelGGL(k_sumsq<double2>, dim3(nb), dim3(kRedThreads), 0, stream(), (const double2*)st->d_amps,
                            st->dim, partial);
     else
-        hipLaunchKernelGGL(k_sumsq<float2>, dim3(nb), dim3(kRedThreads), 0, g_stream, (const float2*)st->d_amps,
+        hipLaunchKernelGGL(k_sumsq<float2>, dim3(nb), dim3(kRedThreads), 0, stream(), (const float2*)st->d_amps,
                            st->dim, partial);
     LQ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(1024), 0, g_stream, (const double*)partial, nb, d_out, 1);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(1024), 0, stream(), (const double*)partial, nb, d_out, 1);
     LQ_LAUNCH_CHECK();
     return LQ_OK;
 }
@@ -302,17 +314,17 @@ int normalize_device(lq_state* st, double* norm_out) {
     if (rc) return rc;
     const unsigned g = blocks_one_per_thread(st->dim);
     if (st->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_scale<double2>, dim3(g), dim3(256), 0, g_stream, (double2*)st->d_amps, st->dim,
+        hipLaunchKernelGGL(k_scale<double2>, dim3(g), dim3(256), 0, stream(), (double2*)st->d_amps, st->dim,
                            (const double*)d);
     else
-        hipLaunchKernelGGL(k_scale<float2>, dim3(g), dim3(256), 0, g_stream, (float2*)st->d_amps, st->dim,
+        hipLaunchKernelGGL(k_scale<float2>, dim3(g), dim3(256), 0, stream(), (float2*)st->d_amps, st->dim,
                            (const double*)d);
     LQ_LAUNCH_CHECK();
     if (norm_out) {
         double* h = host_staging(4);
         if (!h) return LQ_ERR_OUT_OF_MEMORY;
-        LQ_HIP_CHECK(hipMemcpyAsync(h, d, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-        LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(h, d, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
         *norm_out = h[1];
     }
     return LQ_OK;
@@ -321,10 +333,10 @@ int normalize_device(lq_state* st, double* norm_out) {
 static int fill_state(lq_state* st, double value, uint64_t one_index) {
     const unsigned g = blocks_one_per_thread(st->dim);
     if (st->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_fill<double2>, dim3(g), dim3(256), 0, g_stream, (double2*)st->d_amps, st->dim, value,
+        hipLaunchKernelGGL(k_fill<double2>, dim3(g), dim3(256), 0, stream(), (double2*)st->d_amps, st->dim, value,
                            one_index);
     else
-        hipLaunchKernelGGL(k_fill<float2>, dim3(g), dim3(256), 0, g_stream, (float2*)st->d_amps, st->dim, value,
+        hipLaunchKernelGGL(k_fill<float2>, dim3(g), dim3(256), 0, stream(), (float2*)st->d_amps, st->dim, value,
                            one_index);
     LQ_LAUNCH_CHECK();
     return LQ_OK;
@@ -361,7 +373,7 @@ int lq_synchronize(void) {
     LQ_API_GUARD;
     int rc = ensure_init();
     if (rc) return rc;
-    LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     return LQ_OK;
 }
 
@@ -380,7 +392,7 @@ int lq_event_record(int slot) {
         LQ_HIP_CHECK(hipEventCreate(&g_events[slot]));
         g_event_made[slot] = true;
     }
-    LQ_HIP_CHECK(hipEventRecord(g_events[slot], g_stream));
+    LQ_HIP_CHECK(hipEventRecord(g_events[slot], stream()));
     return LQ_OK;
 }
 
@@ -440,7 +452,7 @@ int lq_state_destroy(lq_state* st) {
     LQ_API_GUARD;
     if (!st) return LQ_OK;
     if (st->d_amps) {
-        (void)hipStreamSynchronize(g_stream);
+        (void)hipStreamSynchronize(stream());
         (void)hipFree(st->d_amps);
     }
     st->d_amps = nullptr;
@@ -499,7 +511,7 @@ int lq_state_upload(lq_state* st, const double* amps, uint64_t len, int normaliz
         return LQ_ERR_INVALID_STATE_DIMENSION;
     }
     if (st->dtype == LQ_C128) {
-        LQ_HIP_CHECK(hipMemcpyAsync(st->d_amps, amps, len * 16, hipMemcpyHostToDevice, g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(st->d_amps, amps, len * 16, hipMemcpyHostToDevice, stream()));
     } else {
         // stage f64 on the device, convert to f32 there
         uint64_t count = 2 * len;
@@ -508,13 +520,13 @@ int lq_state_upload(lq_state* st, const double* amps, uint64_t len, int normaliz
             set_error("staging allocation failed");
             return LQ_ERR_OUT_OF_MEMORY;
         }
-        LQ_HIP_CHECK(hipMemcpyAsync(tmp, amps, count * sizeof(double), hipMemcpyHostToDevice, g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(tmp, amps, count * sizeof(double), hipMemcpyHostToDevice, stream()));
         unsigned g = grid_for(count, 256, 16384);
-        hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(256), 0, g_stream, (const double*)tmp, (float*)st->d_amps,
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(256), 0, stream(), (const double*)tmp, (float*)st->d_amps,
                            count);
         LQ_LAUNCH_CHECK();
     }
-    LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     if (normalize) return normalize_device(st, nullptr);
     return LQ_OK;
 }
@@ -529,12 +541,12 @@ int lq_state_download(const lq_state* st, double* amps, uint64_t len) {
         return LQ_ERR_DIMENSION_MISMATCH;
     }
     if (st->dtype == LQ_C128) {
-        LQ_HIP_CHECK(hipMemcpyAsync(amps, st->d_amps, len * 16, hipMemcpyDeviceToHost, g_stream));
-        LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(amps, st->d_amps, len * 16, hipMemcpyDeviceToHost, stream()));
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     } else {
         std::vector<float> tmp(2 * len);
-        LQ_HIP_CHECK(hipMemcpyAsync(tmp.data(), st->d_amps, len * 8, hipMemcpyDeviceToHost, g_stream));
-        LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(tmp.data(), st->d_amps, len * 8, hipMemcpyDeviceToHost, stream()));
+        LQ_HIP_CHECK(hipStreamSynchronize(stream()));
         for (uint64_t i = 0; i < 2 * len; ++i) amps[i] = (double)tmp[i];
     }
     return LQ_OK;
@@ -554,11 +566,11 @@ int lq_state_copy(lq_state* dst, const lq_state* src) {
     }
     const uint64_t bytes = src->dim * src->amp_bytes;
     if (bytes % 16 == 0) {
-        hipLaunchKernelGGL(k_copy16, dim3(blocks_one_per_thread(bytes / 16)), dim3(256), 0, g_stream,
+        hipLaunchKernelGGL(k_copy16, dim3(blocks_one_per_thread(bytes / 16)), dim3(256), 0, stream(),
                            (const copy_unit*)src->d_amps, (copy_unit*)dst->d_amps, bytes / 16);
         LQ_LAUNCH_CHECK();
     } else {
-        LQ_HIP_CHECK(hipMemcpyAsync(dst->d_amps, src->d_amps, bytes, hipMemcpyDeviceToDevice, g_stream));
+        LQ_HIP_CHECK(hipMemcpyAsync(dst->d_amps, src->d_amps, bytes, hipMemcpyDeviceToDevice, stream()));
     }
     return LQ_OK;
 }
@@ -582,8 +594,8 @@ int lq_state_norm_sqr(const lq_state* st, double* out) {
     if (rc) return rc;
     double* h = host_staging(4);
     if (!h) return LQ_ERR_OUT_OF_MEMORY;
-    LQ_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+    LQ_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     *out = h[0];
     return LQ_OK;
 }
@@ -606,18 +618,18 @@ int lq_state_inner_product(const lq_state* a, const lq_state* b, double out[2]) 
     double* d = norm_slot();
     if (!partial || !d) return LQ_ERR_OUT_OF_MEMORY;
     if (a->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_inner<double2>, dim3(nb), dim3(kRedThreads), 0, g_stream, (const double2*)a->d_amps,
+        hipLaunchKernelGGL(k_inner<double2>, dim3(nb), dim3(kRedThreads), 0, stream(), (const double2*)a->d_amps,
                            (const double2*)b->d_amps, a->dim, partial);
     else
-        hipLaunchKernelGGL(k_inner<float2>, dim3(nb), dim3(kRedThreads), 0, g_stream, (const float2*)a->d_amps,
+        hipLaunchKernelGGL(k_inner<float2>, dim3(nb), dim3(kRedThreads), 0, stream(), (const float2*)a->d_amps,
                            (const float2*)b->d_amps, a->dim, partial);
     LQ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_final_sum2, dim3(1), dim3(1024), 0, g_stream, (const double*)partial, nb, d);
+    hipLaunchKernelGGL(k_final_sum2, dim3(1), dim3(1024), 0, stream(), (const double*)partial, nb, d);
     LQ_LAUNCH_CHECK();
     double* h = host_staging(4);
     if (!h) return LQ_ERR_OUT_OF_MEMORY;
-    LQ_HIP_CHECK(hipMemcpyAsync(h, d, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+    LQ_HIP_CHECK(hipMemcpyAsync(h, d, 2 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     out[0] = h[0];
     out[1] = h[1];
     return LQ_OK;
@@ -636,12 +648,12 @@ int lq_state_probabilities(const lq_state* st, double* out, uint64_t len) {
     if (!d) return LQ_ERR_OUT_OF_MEMORY;
     unsigned g = grid_for(st->dim, 256, 16384);
     if (st->dtype == LQ_C128)
-        hipLaunchKernelGGL(k_probs<double2>, dim3(g), dim3(256), 0, g_stream, (const double2*)st->d_amps, st->dim, d);
+        hipLaunchKernelGGL(k_probs<double2>, dim3(g), dim3(256), 0, stream(), (const double2*)st->d_amps, st->dim, d);
     else
-        hipLaunchKernelGGL(k_probs<float2>, dim3(g), dim3(256), 0, g_stream, (const float2*)st->d_amps, st->dim, d);
+        hipLaunchKernelGGL(k_probs<float2>, dim3(g), dim3(256), 0, stream(), (const float2*)st->d_amps, st->dim, d);
     LQ_LAUNCH_CHECK();
-    LQ_HIP_CHECK(hipMemcpyAsync(out, d, st->dim * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    LQ_HIP_CHECK(hipStreamSynchronize(g_stream));
+    LQ_HIP_CHECK(hipMemcpyAsync(out, d, st->dim * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     return LQ_OK;
 }
 
