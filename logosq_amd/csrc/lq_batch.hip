@@ -19,6 +19,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "lq_expect_jit.h"
 #include "lq_fused.h"
 
 namespace lq {
@@ -197,6 +198,11 @@ struct BatchJob {
     std::vector<size_t> lay_off;
     int tile_gx = 0;
     size_t tpart_off = 0;
+    // the same layouts from the kernels specialised at run time (xp; their
+    // coefficient table at coff in the terms buffer) instead of lays
+    bool use_x = false;
+    XPlan xp;
+    size_t coff = 0;
     // shared prefixes: instances are ordered by the step where they leave
     // instance 0's circuit; alive[k] instances take part in step k (the rest
     // still equal instance 0) and alive[steps] = ninst.  Empty: all, always.
@@ -282,14 +288,30 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     j.lays.clear();
     j.lay_off.clear();
     j.tile_gx = 0;
+    j.use_x = false;
+    j.xp = XPlan();
     std::vector<char> tiled;
+    bool diag_tiled = false;
     if (sums.size() == 1 && j.inst_off.empty()) {
+        // one sum for every instance: flip groups from LDS tiles.  The
+        // specialised layout kernels when the policy allows, the batch's
+        // states exceed the caches and every layout compiles; else the
+        // generic tile kernel for states over 64 MiB; else orbit items only.
         std::vector<int> rest;
-        plan_expect_tiles(j.n, j.dtype, sums[0], j.lays, rest);
+        if (expect_jit_wanted_batch(j.n, j.dtype, (uint64_t)j.ninst) && plan_expect_jit(j.n, j.dtype, sums[0], j.xp) &&
+            prepare_expect_jit(j.xp, false, true)) {
+            j.use_x = true;
+            rest = j.xp.rest;
+            diag_tiled = j.xp.diag_in;
+            j.tile_gx = expect_jit_blocks(j.xp, (uint64_t)j.ninst);
+        } else {
+            j.xp = XPlan();
+            plan_expect_tiles(j.n, j.dtype, sums[0], j.lays, rest);
+            if (!j.lays.empty()) j.tile_gx = (int)expect_tile_blocks(j.lays[0].geo, (uint64_t)j.ninst);
+        }
         tiled.assign(sums[0].groups.size(), 0);
         for (size_t g = 0; g < tiled.size(); ++g) tiled[g] = sums[0].groups[g].x != 0;
         for (int g : rest) tiled[g] = 0;
-        if (!j.lays.empty()) j.tile_gx = (int)expect_tile_blocks(j.lays[0].geo, (uint64_t)j.ninst);
     }
     // orbit sets of each distinct sum, their terms and X tables concatenated
     const int maxm = orbit_max_groups(j.dtype);
@@ -300,8 +322,11 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
         std::vector<int> flip;
         int diag = -1;
         for (size_t g = 0; g < sums[k].groups.size(); ++g) {
-            if (sums[k].groups[g].x == 0) diag = (int)g;
-            else if (tiled.empty() || !tiled[g]) flip.push_back((int)g);
+            if (sums[k].groups[g].x == 0) {
+                if (!diag_tiled) diag = (int)g;
+            } else if (tiled.empty() || !tiled[g]) {
+                flip.push_back((int)g);
+            }
         }
         plan_orbits(sums[k], flip, diag, maxm, sets[k]);
     });
@@ -350,8 +375,13 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
     }
     j.xoff = al(std::max<size_t>(nterms, 1) * sizeof(OrbitTerm));
     j.terms_bytes = j.xoff + std::max<size_t>(nx, 1) * sizeof(uint64_t);
+    if (j.use_x) {
+        j.coff = al(j.terms_bytes);
+        j.terms_bytes = j.coff + expect_jit_coef_count(j.xp) * sizeof(double);
+    }
     j.tpart_off = (size_t)std::max(blocks, 1);
-    const size_t tparts = j.lays.size() * (size_t)j.ninst * (size_t)j.tile_gx;
+    const size_t nlay = j.use_x ? j.xp.layouts.size() : j.lays.size();
+    const size_t tparts = nlay * (size_t)j.ninst * (size_t)j.tile_gx;
     int rc = g_items.get(j.items_bytes + 256);
     if (!rc) rc = g_terms.get(j.terms_bytes);
     if (!rc) rc = g_part.get((j.tpart_off + tparts + 1) * sizeof(double));
@@ -370,6 +400,7 @@ static int job_expect(BatchJob& j, const std::vector<PauliSum>& sums, const std:
         std::memcpy(hi + j.lay_off[l] + al(lay.groups.size() * sizeof(ExpTileGroup)), lay.terms.data(),
                     lay.terms.size() * sizeof(ExpTileTerm));
     }
+    if (j.use_x) expect_jit_coefs(j.xp, (double*)((char*)g_hterms.p + j.coff));
     OrbitTerm* ht = (OrbitTerm*)g_hterms.p;
     uint64_t* hx = (uint64_t*)((char*)g_hterms.p + j.xoff);
     for (size_t k = 0; k < sums.size(); ++k) {
@@ -471,7 +502,7 @@ static int job_launch(const BatchJob& j, void* psi) {
 // The expectation part of a job's launch sequence: item and term upload,
 // orbit items, tile layouts, per-instance sums, result copy to g_hout.
 static int expect_launch(const BatchJob& j, void* psi) {
-    if (j.blocks > 0 || !j.lays.empty()) {
+    if (j.blocks > 0 || !j.lays.empty() || j.use_x) {
         LQ_HIP_CHECK(hipMemcpyAsync(g_items.p, g_hitems.p, j.items_bytes, hipMemcpyHostToDevice, stream()));
         LQ_HIP_CHECK(hipMemcpyAsync(g_terms.p, g_hterms.p, j.terms_bytes, hipMemcpyHostToDevice, stream()));
         const int* d_first = (const int*)((char*)g_items.p + j.roff);
@@ -490,6 +521,11 @@ static int expect_launch(const BatchJob& j, void* psi) {
         }
         double* d_tpart = (double*)g_part.p + j.tpart_off;
         auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        if (j.use_x) {
+            int rc = launch_expect_jit(j.xp, psi, (const double*)((const char*)g_terms.p + j.coff), 0,
+                                       (uint64_t)j.ninst, j.dim, j.tile_gx, d_tpart);
+            if (rc) return rc;
+        }
         for (size_t l = 0; l < j.lays.size(); ++l) {
             const ExpLayout& lay = j.lays[l];
             const char* dp = (const char*)g_items.p + j.lay_off[l];
@@ -500,7 +536,8 @@ static int expect_launch(const BatchJob& j, void* psi) {
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_sum_ranges, dim3(j.ninst), dim3(256), 0, stream(), (const double*)g_part.p, d_first,
-                           d_count, (const double*)d_tpart, (int)j.lays.size(), j.tile_gx, j.ninst,
+                           d_count, (const double*)d_tpart,
+                           (int)(j.use_x ? j.xp.layouts.size() : j.lays.size()), j.tile_gx, j.ninst,
                            (double*)g_out.p);
         LQ_LAUNCH_CHECK();
         LQ_HIP_CHECK(hipMemcpyAsync(g_hout.p, g_out.p, j.ninst * sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -547,6 +584,12 @@ static void expect_key(std::vector<uint64_t>& k, const BatchJob& j) {
         k.push_back(((uint64_t)(uint32_t)l.geo.L << 32) | (uint32_t)l.geo.hb);
         for (int h = 0; h < l.geo.hb; ++h) k.push_back((uint64_t)l.geo.hpos[h]);
         k.push_back(((uint64_t)l.groups.size() << 32) | l.terms.size());
+    }
+    k.push_back(j.use_x ? 1 : 0);
+    k.push_back(j.coff);
+    for (const XLayout& l : j.xp.layouts) {
+        k.push_back((uint64_t)(uintptr_t)l.k);
+        k.push_back(((uint64_t)(uint32_t)l.K << 32) | (uint32_t)(l.coef.size() + l.derived.size()));
     }
 }
 
@@ -626,7 +669,7 @@ static int job_run(const BatchJob& j, void* psi, double* energies) {
                          [&] { return job_launch(j, psi); });
     if (rc) return rc;
     LQ_HIP_CHECK(hipStreamSynchronize(stream()));
-    if (j.blocks > 0 || !j.lays.empty()) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
+    if (j.blocks > 0 || !j.lays.empty() || j.use_x) std::memcpy(energies, g_hout.p, j.ninst * sizeof(double));
     else
         for (int i = 0; i < j.ninst; ++i) energies[i] = 0.0;
     return LQ_OK;
@@ -1116,7 +1159,7 @@ static int uneven_run(UnevenJob& u, const lq_gate* const* circuits, const int* n
     LQ_HIP_CHECK(hipStreamSynchronize(stream()));
     const auto t3 = clk::now();
     const double* h = (const double*)g_hout.p;
-    for (int i = 0; i < ninst; ++i) energies[obs_of[i]] = (ex.blocks > 0 || !ex.lays.empty()) ? h[i] : 0.0;
+    for (int i = 0; i < ninst; ++i) energies[obs_of[i]] = (ex.blocks > 0 || !ex.lays.empty() || ex.use_x) ? h[i] : 0.0;
     if (timing) {
         auto ms = [](clk::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
         std::fprintf(stderr, "lq_batch_expectations: %d circuits in %zu pieces, %d rounds: prepare %.3f ms, "

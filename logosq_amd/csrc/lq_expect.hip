@@ -235,6 +235,12 @@ bool expect_tiles_wanted(int n, int dtype) {
     return bytes > (64ull << 20);
 }
 
+bool expect_tiles_batch_wanted(int n, int dtype, uint64_t ninst) {
+    if (g_exp_tiles == LQ_EXPECT_TILES_OFF) return false;
+    if (g_exp_tiles == LQ_EXPECT_TILES_ON) return true;
+    return (double)ninst * std::ldexp(dtype == LQ_C128 ? 16.0 : 8.0, n) > (double)(256ull << 20);
+}
+
 void plan_expect_tiles(int n, int dtype, const PauliSum& ps, std::vector<ExpLayout>& layouts,
                        std::vector<int>& rest) {
     layouts.clear();

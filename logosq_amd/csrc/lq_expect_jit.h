@@ -55,6 +55,8 @@ struct XPlan {
 // The policy wants the specialised kernels for an n-qubit state (run-time
 // specialisation not off, tiles wanted at this size, one wavefront per tile).
 bool expect_jit_wanted(int n, int dtype);
+// The same for a batch of ninst n-qubit states sharing one sum.
+bool expect_jit_wanted_batch(int n, int dtype, uint64_t ninst);
 // Plan the layouts of a compiled sum; false when the sum does not suit them.
 bool plan_expect_jit(int n, int dtype, const PauliSum& ps, XPlan& xp);
 // Attach compiled kernels (compiling as the policy allows; force: at first

@@ -212,6 +212,12 @@ bool expect_jit_wanted(int n, int dtype) {
     return expect_tiles_wanted(n, dtype);
 }
 
+bool expect_jit_wanted_batch(int n, int dtype, uint64_t ninst) {
+    if (jit_mode() == LQ_JIT_OFF) return false;
+    if (n < window_bits(dtype) + 6) return false;
+    return expect_tiles_batch_wanted(n, dtype, ninst);
+}
+
 // Plan the layouts of a compiled sum on an n-qubit state.  False when the
 // sum does not suit the specialised kernels (no flip group fits a layout, or
 // too many terms).

@@ -81,6 +81,9 @@ struct ExpLayout {
 // all of them, for states under 8 qubits or when tiles are not wanted for
 // this size) are returned in rest (indices into ps.groups).
 bool expect_tiles_wanted(int n, int dtype);
+// The same policy for a batch of ninst such states (auto: their total size
+// beyond the 256 MB Infinity Cache).
+bool expect_tiles_batch_wanted(int n, int dtype, uint64_t ninst);
 void plan_expect_tiles(int n, int dtype, const PauliSum& ps, std::vector<ExpLayout>& layouts, std::vector<int>& rest);
 size_t expect_tile_lds(int dtype, const ExpGeom& geo);
 unsigned expect_tile_blocks(const ExpGeom& geo, uint64_t ninst);
