@@ -716,7 +716,7 @@ int param_shift_batched(int n, int dtype, const lq_gate* tmpl, int num_gates, co
     int rc = gates_to_prims(n, bound.data(), num_gates, base);
     if (rc) return rc;
     Plan plan;
-    plan_passes(n, base, default_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
+    plan_passes(n, base, batch_tile_bits(n, dtype), default_low_bits(n, dtype), plan);
     jit_prepare(plan, dtype, (uint64_t)ninst);
     std::vector<int> fp = first_prims(base, num_gates);
     // which gates use each parameter slot
@@ -915,7 +915,7 @@ static constexpr size_t kPlanCacheMax = 512;
 
 static int cached_plan(std::vector<int64_t> key, int n, int dtype, const lq_gate* gates, int ng, uint64_t ninst,
                        CachedPlan** out) {
-    const int tb = default_tile_bits(n, dtype), lb = default_low_bits(n, dtype);
+    const int tb = batch_tile_bits(n, dtype), lb = default_low_bits(n, dtype);
     key.push_back(dtype);
     key.push_back(tb);
     key.push_back(lb);

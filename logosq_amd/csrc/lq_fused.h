@@ -47,6 +47,8 @@ int gates_to_prims(int n, const lq_gate* gates, int count, std::vector<Prim>& ou
 void plan_passes(int n, const std::vector<Prim>& prims, int tile_bits, int low_bits, Plan& plan);
 // Tile size choice for a state (bits), given the amplitude size.
 int default_tile_bits(int n, int dtype);
+// The tile size for batches of states (many gates per pass).
+int batch_tile_bits(int n, int dtype);
 int default_low_bits(int n, int dtype);
 
 // Launch one tile pass over `ninst` states laid out `inst_stride` amplitudes
