@@ -7,7 +7,11 @@
 //   - batches of unrelated circuits of uneven qubit counts.
 // Accumulation is f64 for both amplitude types, in a fixed order.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
@@ -257,21 +261,81 @@ static int job_ops(BatchJob& j, TileOp** out) {
     return LQ_OK;
 }
 
-// Run fn(i) for i in [0, count) on up to 8 host threads (inline when small).
+// Host worker threads for the preparation of batches, created at the first
+// use and kept (a batch call hands out work several times; creating threads
+// for each costs more than the work): indices go out through an atomic
+// counter, and the calling thread takes its share.  Calls come one at a time
+// (the library lock).
+class HostPool {
+  public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    void run(int count, const std::function<void(int)>& fn) {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (th_.empty()) {
+            const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+            const unsigned nw = std::min(7u, hw > 1 ? hw - 1 : 0u);
+            for (unsigned t = 0; t < nw; ++t) th_.emplace_back([this] { work(); });
+        }
+        fn_ = &fn;
+        count_ = count;
+        next_.store(0);
+        ++gen_;
+        lk.unlock();
+        cv_.notify_all();
+        for (int i; (i = next_.fetch_add(1)) < count;) fn(i);
+        lk.lock();
+        done_.wait(lk, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+
+  private:
+    void work() {
+        long seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            if (!fn_) continue;
+            const std::function<void(int)>* fn = fn_;
+            const int count = count_;
+            ++busy_;
+            lk.unlock();
+            for (int i; (i = next_.fetch_add(1)) < count;) (*fn)(i);
+            lk.lock();
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::vector<std::thread> th_;
+    const std::function<void(int)>* fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int count_ = 0, busy_ = 0;
+    long gen_ = 0;
+    bool stop_ = false;
+};
+
+// Run fn(i) for i in [0, count) on the host pool (inline when small).
 template <typename F>
 static void parallel_for(int count, F&& fn) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nth = (int)std::min<unsigned>({8u, hw, (unsigned)std::max(count / 16, 1)});
-    if (nth <= 1) {
+    if (count < 32) {
         for (int i = 0; i < count; ++i) fn(i);
         return;
     }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nth; ++t)
-        th.emplace_back([&, t] {
-            for (int i = t; i < count; i += nth) fn(i);
-        });
-    for (auto& x : th) x.join();
+    const std::function<void(int)> f = fn;
+    HostPool::get().run(count, f);
 }
 
 // Expectation work of the batch (obs_of[i] indexes `sums`): with one sum
