@@ -156,11 +156,11 @@ def time_kernels(lq, st, gates, reps=1):
     return ms.value / 1e3
 
 
-def num_passes(lq, n, dtype, descs):
+def num_passes(lq, n, dtype, descs, tile_bits=0):
     from logosq_amd import abi
     order = (ctypes.c_int * len(descs))()
     npass = ctypes.c_int()
-    lq._lib.call("lq_plan_fused", n, dtype, abi.gate_array(descs), len(descs), 0, 0, order, None,
+    lq._lib.call("lq_plan_fused", n, dtype, abi.gate_array(descs), len(descs), tile_bits, 0, order, None,
                  ctypes.byref(npass))
     return npass.value
 
@@ -265,7 +265,9 @@ def uneven_batch_bench(lq, reps=3):
     by_n = {}
     for n, gates, codes, coeffs in items:
         if n not in by_n:
-            by_n[n] = {dt: num_passes(lq, n, dt, gates) for dt in (0, 1)}
+            # batches plan complex64 states with 2^12-amplitude tiles (the
+            # library's batch default), complex128 with the default 2^12
+            by_n[n] = {0: num_passes(lq, n, 0, gates), 1: num_passes(lq, n, 1, gates, min(12, n))}
     reads = []
     for n, gates, codes, coeffs in items:
         masks = set()
