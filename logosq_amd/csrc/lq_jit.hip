@@ -183,11 +183,13 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     const bool single = dfirst && p.group_count == 1;
     const bool dlast = direct_groups && p.group_count >= 2 && lanes_low(p.group_count - 1);
     // a direct first group that reads the state is issued one tile ahead
-    // when the registers allow: blocks of 256 threads run two waves per SIMD
-    // (256 VGPRs each); at 512 threads (K = 13) the extra 16 amplitudes per
-    // thread cost more than the overlap gains (complex64 brickwork 127 ->
-    // 138 ms)
-    const bool pipe = dfirst && !init && NT <= 256;
+    // when the registers allow: at two waves per SIMD (256 VGPRs each:
+    // blocks of 256 threads, or complex128 2^13 tiles whose 128 KiB of LDS
+    // leave one block of 512 threads per CU; brickwork 264 -> 240 ms at
+    // 13 / 5 bits); complex64 2^13 tiles run two such blocks per CU, four
+    // waves per SIMD, where the extra 16 amplitudes per thread cost more
+    // than the overlap gains (complex64 brickwork 127 -> 138 ms)
+    const bool pipe = dfirst && !init && (NT <= 256 || dtype == LQ_C128);
     // gather (which = 1) / scatter (2) slot flips of group gi, as a lambda's text
     auto flips_of = [&](int gi, int which) {
         const TileGroup& G = groups[p.group_first + gi];
