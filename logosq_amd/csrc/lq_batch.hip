@@ -1322,7 +1322,12 @@ extern "C" int lq_batch_expectations(int count, const int* num_qubits, const lq_
     // `budget` bytes
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) freeb = (size_t)8 << 30;
-    const size_t budget = std::max<size_t>(std::min<size_t>(freeb / 2, (size_t)96 << 30), (size_t)1 << 20);
+    size_t budget = std::max<size_t>(std::min<size_t>(freeb / 2, (size_t)96 << 30), (size_t)1 << 20);
+    // LQ_BATCH_BUDGET_MIB caps it (tests of the split into several jobs)
+    if (const char* e = std::getenv("LQ_BATCH_BUDGET_MIB")) {
+        const long long mib = std::atoll(e);
+        if (mib > 0) budget = std::min<size_t>(budget, (size_t)mib << 20);
+    }
     std::vector<UnevenPiece> pieces;
     for (const auto& [key, members] : order) {
         const int n = num_qubits[(*members)[0]];

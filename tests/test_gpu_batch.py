@@ -3,6 +3,7 @@ parameter-shift circuits in one batch (energy and gradient <= 1e-10 for
 complex128, <= 1e-5 for complex64 with f64 reductions), and batches of
 circuits with uneven qubit counts and their own Pauli sums."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -104,6 +105,26 @@ def test_uneven_batch_one_sequence_every_call(hip, dtype, tol):
         err = np.abs(got - want)
         say("  with a MatrixGate, call %d: max |dE| %.3e" % (call, err.max()))
         assert err.max() <= tol, (call, int(np.argmax(err)))
+
+
+def test_uneven_batch_split_into_jobs(hip):
+    """A memory budget smaller than the batch (LQ_BATCH_BUDGET_MIB=1): the
+    circuits run as several launch sequences, a structure's instances split
+    across pieces, states over the budget alone; every energy still equals
+    the oracle's, and the same as one unsplit call."""
+    say("uneven batch split by a 1 MiB budget")
+    items = algorithms.uneven_batch_workload(count=30, nmin=8, nmax=17, layers=2, terms=6, seed=31)
+    whole = lq.batch_expectations(items, lq.C128)
+    os.environ["LQ_BATCH_BUDGET_MIB"] = "1"
+    try:
+        split = [lq.batch_expectations(items, lq.C128) for _ in range(3)]
+    finally:
+        del os.environ["LQ_BATCH_BUDGET_MIB"]
+    want = np.array([oracle.pauli_sum(oracle.execute(oracle.zero_state(n), n, gates), n, codes, coeffs)
+                     for n, gates, codes, coeffs in items])
+    for e in [whole] + split:
+        assert np.max(np.abs(e - want)) <= 1e-10
+    assert np.max(np.abs(split[0] - whole)) <= 1e-13
 
 
 @pytest.mark.parametrize("n,layers", [(3, 2), (8, 3), (14, 2)])
