@@ -21,6 +21,7 @@ struct Prim {
     uint64_t ds;        // bits the op depends on diagonally (controls, selectors)
     int32_t src;        // index of the gate this primitive came from
     int32_t mclass;     // kTPairReal for RY/CRY, kTPairRX for RX/CRX, else kTPairMat
+    int32_t dclass;     // kTDiagNeg for Z/CZ, else 0
 };
 
 struct JitKernel;  // a pass kernel compiled at run time for one pass structure

@@ -85,6 +85,7 @@ int gates_to_prims(int n, const lq_gate* gates, int count, std::vector<Prim>& ou
                 d[0] = -1.0;
                 d[1] = 0.0;
                 out.push_back(diag_prim(bmask(n, q[0]), -1, -1, d));
+                out.back().dclass = kTDiagNeg;
                 break;
             case LQ_GATE_H: {
                 double s = 1.0 / std::sqrt(2.0);  // single_qubit.rs:237
@@ -137,6 +138,7 @@ int gates_to_prims(int n, const lq_gate* gates, int count, std::vector<Prim>& ou
                 d[0] = -1.0;
                 d[1] = 0.0;
                 out.push_back(diag_prim(bmask(n, q[0]) | bmask(n, q[1]), -1, -1, d));
+                out.back().dclass = kTDiagNeg;
                 break;
             case LQ_GATE_CPHASE:
             case LQ_GATE_CP:
@@ -240,6 +242,7 @@ static TileOp to_tile_op(const Prim& p, int L, const int* hpos, int hb) {
     t.sel1 = p.sel1;
     t.ctrl = p.ctrl;
     t.mclass = p.mclass;
+    t.dclass = p.dclass;
     std::memcpy(t.m, p.m, sizeof(t.m));
     return t;
 }

@@ -53,6 +53,8 @@ static constexpr uint64_t kStreamBytes = 1ull << 30;
 // LQ_JIT_DIRECT=0 in the environment keeps every pass on LDS tile loads and
 // stores, for comparison)
 static const bool direct_groups = !(std::getenv("LQ_JIT_DIRECT") && std::getenv("LQ_JIT_DIRECT")[0] == '0');
+// Z / CZ as sign flips (LQ_JIT_SIGN=0: as complex phases like other diagonals)
+static const bool sign_flips = !(std::getenv("LQ_JIT_SIGN") && std::getenv("LQ_JIT_SIGN")[0] == '0');
 static constexpr size_t kJitMaxStructures = 4096;
 static std::unordered_map<std::string, std::unique_ptr<JitKernel>> g_jit;
 static long long g_jit_compiled = 0, g_jit_failed = 0, g_jit_launches = 0;
@@ -74,12 +76,19 @@ static constexpr int jit_threads(int K) { return K <= 12 ? 256 : (1 << (K - 4));
 //   more becomes one 16-entry phase table per tile: sixteen threads form the
 //   products (in f64) into LDS while the tile loads, and the group applies
 //   the whole run with one complex multiply per amplitude.
+// * Sign ops (Z, CZ: kTDiagNeg) move the same way and are applied as sign
+//   flips: an op whose controls lie inside the subcube negates its slots at
+//   compile time; one with controls outside flips the sign bits of its slots
+//   by a per-thread mask (an XOR, no multiply and no table entry).
 static constexpr int kMaxRuns = 32;
 
 struct GroupPlan {
     std::vector<int> fold;            // per op of the group: 0, 1 gather, 2 scatter
-    std::vector<std::pair<int, int>> items;  // (0, op index), (1, run id) or (2, 0: the thread phase), in order
+    // (0, op index), (1, run id), (2, 0: the thread phase) or (3, sign set id), in order
+    std::vector<std::pair<int, int>> items;
     std::vector<int> uphase;          // diagonal ops on outside bits only, merged into one thread phase
+    std::vector<int> uneg;            // sign ops on outside bits only, folded into the thread phase
+    std::vector<std::vector<int>> signs;  // sign sets: ops applied together as sign flips
 };
 
 // init: the pass starts from |0...0> (j_tile_zero in place of the tile
@@ -109,6 +118,11 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     auto outside_diag = [&](const TileOp& t) {
         return t.kind == kTDiag && t.cin == 0 && t.s0p < 0 && t.s1p < 0 && (t.cout != 0 || t.osel0 >= 0 || t.osel1 >= 0);
     };
+    // negates where its controls are set (checked on the values as well)
+    auto sign_op = [&](const TileOp& t) {
+        return sign_flips && t.kind == kTDiag && t.dclass == kTDiagNeg && t.sel0 < 0 && t.sel1 < 0 && t.ctrl != 0 &&
+               t.m[0] == -1.0 && t.m[1] == 0.0;
+    };
     std::vector<std::vector<int>> runs;  // op indices (pass-relative) of each merged run
     std::vector<GroupPlan> gp(p.group_count);
     for (int gi = 0; gi < p.group_count; ++gi) {
@@ -129,8 +143,9 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             if (foldable(t) && !(seen & (1 << t.sp0))) P.fold[o - o0] = 2;
             else seen |= touch(t);
         }
-        // pending local diagonals sink past non-diagonal ops on other bits
-        std::vector<int> pend;
+        // pending local diagonals and sign ops sink past non-diagonal ops on
+        // other bits
+        std::vector<int> pend, pend_sign;
         int pend_dep = 0;
         auto flush = [&] {
             if (pend.size() >= 2 && (int)runs.size() < kMaxRuns) {
@@ -139,13 +154,23 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             } else {
                 for (int o : pend) P.items.emplace_back(0, o);
             }
+            if (!pend_sign.empty()) {
+                P.items.emplace_back(3, (int)P.signs.size());
+                P.signs.push_back(pend_sign);
+            }
             pend.clear();
+            pend_sign.clear();
             pend_dep = 0;
         };
         for (int o = o0; o < o1; ++o) {
             if (P.fold[o - o0]) continue;
             const TileOp& t = ops[p.op_first + o];
-            if (outside_diag(t)) {
+            if (sign_op(t) && t.cin == 0) {
+                P.uneg.push_back(o);  // commutes with every op of the group
+            } else if (sign_op(t)) {
+                pend_sign.push_back(o);
+                pend_dep |= touch(t);
+            } else if (outside_diag(t)) {
                 P.uphase.push_back(o);  // commutes with every op of the group
             } else if (local_diag(t)) {
                 pend.push_back(o);
@@ -160,11 +185,21 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             }
         }
         flush();
-        if (P.uphase.size() >= 2) {
+        // outside sign ops: into the thread phase when there is one, else
+        // into a sign set (their flag flips all sixteen slots)
+        if (P.uphase.size() + (P.uphase.empty() ? 0 : P.uneg.size()) >= 2) {
             P.items.emplace_back(2, 0);
         } else {
             for (int o : P.uphase) P.items.emplace_back(0, o);
             P.uphase.clear();
+            if (!P.uneg.empty()) {
+                if (P.signs.empty()) {
+                    P.items.emplace_back(3, 0);
+                    P.signs.emplace_back();
+                }
+                P.signs.back().insert(P.signs.back().end(), P.uneg.begin(), P.uneg.end());
+                P.uneg.clear();
+            }
         }
     }
 
@@ -300,7 +335,47 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
                       << (t.osel1 >= 0 ? gbit(t.osel1) : std::string("0")) << " << 1)";
                     s << "        j_uphase_mul(pr, pi, " << on.str() << ", " << k.str() << ", op[" << o << "]);\n";
                 }
+                for (int o : P.uneg) {
+                    const TileOp& t = ops[p.op_first + o];
+                    s << "        j_uphase_neg(pr, pi, (gb & 0x" << std::hex << t.cout << "ull) == 0x" << t.cout
+                      << std::dec << "ull);\n";
+                }
                 s << "        j_uphase_apply<T>(xr, xi, pr, pi);\n      }\n";
+                continue;
+            }
+            if (what == 3) {
+                // slot c is negated by every op of the set whose in-subcube
+                // controls it has: at compile time for ops with no outside
+                // controls, else by the op's per-thread sign-bit flag
+                // (the flags of ops on outside bits only, which flip every
+                // slot, are combined first into nu)
+                const std::vector<int>& set = P.signs[idx];
+                s << "{\n";
+                std::vector<int> flag(set.size(), -1);
+                int nflags = 0;
+                std::string nu;
+                for (size_t k = 0; k < set.size(); ++k) {
+                    const TileOp& t = ops[p.op_first + set[k]];
+                    if (!t.cout) continue;
+                    flag[k] = nflags++;
+                    s << "        const uint32_t n" << flag[k] << " = (gb & 0x" << std::hex << t.cout << "ull) == 0x"
+                      << t.cout << std::dec << "ull ? 0x80000000u : 0u;\n";
+                    if (!t.cin) nu += (nu.empty() ? "n" : " ^ n") + std::to_string(flag[k]);
+                }
+                if (!nu.empty()) s << "        const uint32_t nu = " << nu << ";\n";
+                for (int c = 0; c < 16; ++c) {
+                    bool neg = false;
+                    std::string f = nu.empty() ? "" : "nu";
+                    for (size_t k = 0; k < set.size(); ++k) {
+                        const TileOp& t = ops[p.op_first + set[k]];
+                        if ((c & t.cin) != t.cin || (flag[k] >= 0 && !t.cin)) continue;
+                        if (flag[k] < 0) neg = !neg;
+                        else f += (f.empty() ? "n" : " ^ n") + std::to_string(flag[k]);
+                    }
+                    if (neg) s << "        xr[" << c << "] = -xr[" << c << "]; xi[" << c << "] = -xi[" << c << "];\n";
+                    if (!f.empty()) s << "        j_flip(xr[" << c << "], xi[" << c << "], " << f << ");\n";
+                }
+                s << "      }\n";
                 continue;
             }
             if (what == 1) {

@@ -307,6 +307,25 @@ __device__ __forceinline__ void j_uphase_mul(double& pr, double& pi, bool on, in
     pr = nr;
 }
 
+// A sign op on outside bits only, in the thread phase: -(pr, pi) when on.
+__device__ __forceinline__ void j_uphase_neg(double& pr, double& pi, bool on) {
+    pr = on ? -pr : pr;
+    pi = on ? -pi : pi;
+}
+
+// Negate (r, i) when f = 0x80000000 and keep it when f = 0: an XOR of the
+// sign bits (the high word's for complex128).
+__device__ __forceinline__ void j_flip(double& r, double& i, uint32_t f) {
+    const uint64_t m = (uint64_t)f << 32;
+    r = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, r) ^ m);
+    i = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, i) ^ m);
+}
+
+__device__ __forceinline__ void j_flip(float& r, float& i, uint32_t f) {
+    r = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, r) ^ f);
+    i = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, i) ^ f);
+}
+
 template <typename T>
 __device__ __forceinline__ void j_uphase_apply(T* xr, T* xi, double pr, double pi) {
     const T r = (T)pr, i = (T)pi;

@@ -34,6 +34,7 @@ struct TileOp {
     int32_t s0p, s1p;   // grouped kernel: subcube positions of selectors inside S (-1: none)
     int32_t cin;        // grouped kernel: controls inside S, as a mask of subcube positions
     int32_t mclass;     // pair ops: kTPairReal / kTPairRX when the gate kind guarantees the structure
+    int32_t dclass;     // diagonal ops: kTDiagNeg when the gate kind guarantees the structure, else 0
 
     uint64_t ctrl;      // global bits that must be 1 (controls, fixed bits)
     uint64_t cout;      // grouped kernel: the controls outside S
@@ -43,6 +44,9 @@ struct TileOp {
 // Specialised pair kinds used by the grouped kernel (classified at plan time).
 constexpr int32_t kTPairReal = 7;  // all four entries real (RY, H-free real rotations)
 constexpr int32_t kTPairRX = 8;    // real diagonal, imaginary off-diagonal (RX)
+// Diagonal ops that negate the amplitudes whose controls are all set and
+// leave the others (Z, CZ: m = -1 + 0i, no selectors), for every instance.
+constexpr int32_t kTDiagNeg = 1;
 
 constexpr int kMaxTileHigh = 28;
 constexpr int kGroupBits = 4;  // a group acts on at most 4 tile bits (16-amplitude subcubes)

@@ -58,6 +58,48 @@ def test_jit_random_circuits(hip, jit_mode, n):
     assert np.max(np.abs(got64 - gen64)) <= 1e-6
 
 
+def sign_heavy_circuit(n, layers, rng):
+    """RY / RZ layers with CZ rings, Z gates and CPs between them: Z and CZ
+    become sign flips in the specialised kernels (controls inside the
+    subcube: compile-time negations; outside: per-thread flags), beside
+    phase-table runs and thread phases of the other diagonals."""
+    from tests.test_planner import G
+    out = []
+    for _ in range(layers):
+        for q in range(n):
+            out.append(G(abi.RY, [q], [rng.uniform(0, 6.28)]))
+        for q in range(n):
+            out.append(G(abi.CZ, [q, (q + 1) % n]))
+        for q in rng.choice(n, n // 3, replace=False):
+            out.append(G(abi.Z, [int(q)]))
+        for q in range(n):
+            out.append(G(abi.RZ, [q], [rng.uniform(0, 6.28)]))
+        a, b = rng.choice(n, 2, replace=False)
+        out.append(G(abi.CP, [int(a), int(b)], [rng.uniform(0, 6.28)]))
+        a, b = rng.choice(n, 2, replace=False)
+        out.append(G(abi.CZ, [int(a), int(b)]))
+    return out
+
+
+@pytest.mark.parametrize("n", [13, 16])
+def test_jit_sign_ops(hip, jit_mode, n):
+    say("jit sign ops n=%d" % n)
+    rng = np.random.default_rng(4200 + n)
+    gates = sign_heavy_circuit(n, 6, rng)
+    psi = bf.random_state(n, rng)
+    want = oracle.execute(psi, n, gates)
+    jit_mode(jit.OFF)
+    gen = run(psi, n, gates, lq.C128, abi.EXEC_FUSED)
+    jit_mode(jit.ALWAYS)
+    s0 = jit.jit_stats()
+    got = run(psi, n, gates, lq.C128, abi.EXEC_FUSED)
+    got64 = run(psi, n, gates, lq.C64, abi.EXEC_FUSED)
+    assert _launched(s0) > 0
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert np.max(np.abs(got - gen)) <= 1e-13
+    assert np.max(np.abs(got64 - want)) <= 2e-6
+
+
 def test_jit_brickwork_22(hip, jit_mode):
     """Brickwork at 22 qubits (1024 tiles per pass): the repeat policy runs
     the generic kernel the first time and the specialised kernels from the

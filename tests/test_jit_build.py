@@ -32,3 +32,15 @@ def test_jit_compiles_every_op_kind():
     rng = np.random.default_rng(3)
     gates = random_circuit(13, 120, rng, with_dense=False)
     assert jit.compile_plan(13, gates, abi.LQ_C128) >= 1
+
+
+@pytest.mark.parametrize("dtype", [abi.LQ_C128, abi.LQ_C64])
+def test_jit_compiles_sign_ops(dtype):
+    """Z and CZ with controls inside and outside the group subcubes (sign
+    flips at compile time and by per-thread flags) compile."""
+    import numpy as np
+    from tests.test_gpu_jit import sign_heavy_circuit
+    gates = sign_heavy_circuit(14, 3, np.random.default_rng(5))
+    before = jit.jit_stats()
+    assert jit.compile_plan(14, gates, dtype) >= 1
+    assert jit.jit_stats()["failed"] == before["failed"]
