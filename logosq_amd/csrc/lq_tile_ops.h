@@ -300,8 +300,16 @@ __device__ __forceinline__ void j_phase_mul(double& pr, double& pi, int ts, cons
 // Diagonal ops that depend only on bits outside the subcube commute with
 // every op of the group: (pr, pi) *= phase m[k] when the op applies to the
 // thread (its outside controls set); the product is applied once.
+// The four phases are read as block-uniform values and the thread's one is
+// selected in registers: an index into op.m by the per-thread k would be a
+// per-lane memory load in the middle of the group.
 __device__ __forceinline__ void j_uphase_mul(double& pr, double& pi, bool on, int k, const TileOp& op) {
-    const double dr = on ? op.m[2 * k] : 1.0, di = on ? op.m[2 * k + 1] : 0.0;
+    const double r0 = op.m[0], i0 = op.m[1], r1 = op.m[2], i1 = op.m[3];
+    const double r2 = op.m[4], i2 = op.m[5], r3 = op.m[6], i3 = op.m[7];
+    const bool b0 = (k & 1) != 0, b1 = (k & 2) != 0;
+    const double sr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
+    const double si = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
+    const double dr = on ? sr : 1.0, di = on ? si : 0.0;
     const double nr = pr * dr - pi * di;
     pi = pr * di + pi * dr;
     pr = nr;
