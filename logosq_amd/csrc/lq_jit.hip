@@ -66,7 +66,7 @@ static constexpr int jit_threads(int K) { return K <= 12 ? 256 : (1 << (K - 4));
 // and every op's kind, positions and masks.  Op i of the pass reads its
 // matrix from ops[i].
 //
-// Within a group, two rewrites (both exact up to rounding):
+// Within a group, three rewrites (all exact up to rounding):
 // * X gates with their controls outside the subcube and nothing but other
 //   such gates on their bit before (after) them in the group are done by
 //   flipping the gather (scatter) slots of the threads whose controls are
@@ -225,6 +225,12 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     // waves per SIMD, where the extra 16 amplitudes per thread cost more
     // than the overlap gains (complex64 brickwork 127 -> 138 ms)
     const bool pipe = dfirst && !init && (NT <= 256 || dtype == LQ_C128);
+    // a pass of one direct group issues the next tile's gather before its
+    // stores (the vector memory counter retires in order, so waiting for
+    // loads issued after stores waits for the stores too); in longer passes
+    // the gather follows the first group (issuing it before the group costs
+    // 64 register moves per tile and gained nothing)
+    const bool early = single;
     // gather (which = 1) / scatter (2) slot flips of group gi, as a lambda's text
     auto flips_of = [&](int gi, int which) {
         const TileGroup& G = groups[p.group_first + gi];
@@ -261,21 +267,23 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
     s << "  j_pass_begin<K, HB, NT>(hoff, p);\n";
     // each block walks a contiguous range of tiles: consecutive tiles share
     // an instance, so its op table stays in the scalar cache
-    s << "  const uint64_t total = ninst * p.ntiles;\n";
+    // p.ntiles is a power of two: instance and tile index by shift and mask
+    s << "  const uint64_t total = ninst * p.ntiles, tmask = p.ntiles - 1ull;\n";
+    s << "  const uint32_t tsh = (uint32_t)__builtin_ctzll(p.ntiles);\n";
     s << "  const uint64_t per = (total + gridDim.x - 1) / gridDim.x;\n";
     s << "  const uint64_t w0 = (uint64_t)blockIdx.x * per, w1 = w0 + per < total ? w0 + per : total;\n";
     if (!runs.empty()) s << "  uint32_t tab_inst = 0xffffffffu;\n";
     const char* sm = stream ? "true" : "false";
     if (pipe) {
         s << "  V pre[16];\n  const auto gf0 = " << flips_of(0, 1) << ";\n";
-        s << "  if (w0 < w1) {\n    const uint32_t in0 = __builtin_amdgcn_readfirstlane((uint32_t)(w0 / p.ntiles));\n";
+        s << "  if (w0 < w1) {\n    const uint32_t in0 = __builtin_amdgcn_readfirstlane((uint32_t)(w0 >> tsh));\n";
         s << "    j_first_fetch<K, HB, NT, " << pos_of(0) << ", " << sm
-          << ">(pre, psi + (uint64_t)in0 * inst_stride, j_tile_base<K, HB>(w0 - (uint64_t)in0 * p.ntiles, p), hoff, p, "
+          << ">(pre, psi + (uint64_t)in0 * inst_stride, j_tile_base<K, HB>(w0 & tmask, p), hoff, p, "
              "gf0);\n  }\n";
     }
     s << "  for (uint64_t w = w0; w < w1; ++w) {\n";
-    s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w / p.ntiles));\n";
-    s << "    const uint64_t base = j_tile_base<K, HB>(w - (uint64_t)inst * p.ntiles, p);\n";
+    s << "    const uint32_t inst = __builtin_amdgcn_readfirstlane((uint32_t)(w >> tsh));\n";
+    s << "    const uint64_t base = j_tile_base<K, HB>(w & tmask, p);\n";
     s << "    V* st = psi + (uint64_t)inst * inst_stride;\n";
     s << "    const TileOp* op = ops + (uint64_t)inst * ops_stride + p.op_first;\n";
     if (!runs.empty()) {
@@ -312,8 +320,8 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
         const std::string pos = std::to_string(G.spos[0]) + ", " + std::to_string(G.spos[1]) + ", " +
                                 std::to_string(G.spos[2]) + ", " + std::to_string(G.spos[3]);
         if (gi == 0 && pipe)
-            s << "    j_first_apply<K, HB, NT, " << pos << ", " << sm << ", " << (single ? "true" : "false")
-              << ", T>(tile, st, pre, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
+            s << "    j_first_apply<K, HB, NT, " << pos << ", " << sm << ", " << (single ? "true" : "false") << ", "
+              << (early ? "true" : "false") << ", T>(tile, st, pre, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
         else if (gi == 0 && dfirst)
             s << "    j_group_first<K, HB, NT, " << pos << ", " << sm << ", " << (init ? "true" : "false") << ", "
               << (single ? "true" : "false") << ", T>(tile, st, base, hoff, p, [&](T* xr, T* xi, uint64_t gb) {\n";
@@ -410,13 +418,13 @@ static std::string pass_source(const TilePass& p, const TileOp* ops, const TileG
             }
         }
         if (gi == 0 && pipe) {
-            s << "    }, " << flips(2) << ");\n";
-            // the next tile's first gather, in flight while this tile's other groups run
-            s << "    if (w + 1 < w1) {\n      const uint32_t in1 = __builtin_amdgcn_readfirstlane((uint32_t)((w + 1) / "
-                 "p.ntiles));\n";
-            s << "      j_first_fetch<K, HB, NT, " << pos << ", " << sm
-              << ">(pre, psi + (uint64_t)in1 * inst_stride, j_tile_base<K, HB>(w + 1 - (uint64_t)in1 * p.ntiles, p), "
-                 "hoff, p, gf0);\n    }\n";
+            // the next tile's first gather, issued once this tile's is in
+            // registers: in flight while this tile's groups run
+            s << "    }, " << flips(2) << ", [&] {\n";
+            s << "      if (w + 1 < w1) {\n        const uint32_t in1 = __builtin_amdgcn_readfirstlane((uint32_t)((w + 1) >> tsh));\n";
+            s << "        j_first_fetch<K, HB, NT, " << pos << ", " << sm
+              << ">(pre, psi + (uint64_t)in1 * inst_stride, j_tile_base<K, HB>((w + 1) & tmask, p), "
+                 "hoff, p, gf0);\n      }\n    });\n";
         } else {
             s << "    }, " << flips(1) << ", " << flips(2) << ");\n";
         }
@@ -540,6 +548,17 @@ static int jit_prepare_impl(Plan& plan, int dtype, uint64_t ninst, bool load, bo
         if (s.normalize || !jit_eligible(s.pass, ninst, any_size || g_jit_mode == LQ_JIT_ALWAYS)) continue;
         std::string src = pass_source(s.pass, plan.ops.data(), plan.groups.data(), dtype,
                                       init_first && &s == &plan.steps[0], stream);
+        if (const char* dir = std::getenv("LQ_JIT_DUMP")) {
+            // which kernel text each step of the plan uses, for reading
+            if (FILE* f = std::fopen((std::string(dir) + "/steps.txt").c_str(), "a")) {
+                std::fprintf(f, "n=%d step %d L=%d hb=%d groups=%d ops=%d lq_pass_%zu hpos", plan.n,
+                             (int)(&s - plan.steps.data()), s.pass.L, s.pass.hb, s.pass.group_count,
+                             s.pass.op_count, std::hash<std::string>{}(src));
+                for (int j = 0; j < s.pass.hb; ++j) std::fprintf(f, " %d", s.pass.hpos[j]);
+                std::fputc('\n', f);
+                std::fclose(f);
+            }
+        }
         auto it = g_jit.find(src);
         if (it == g_jit.end()) {
             if (g_jit.size() >= kJitMaxStructures) continue;  // bounded cache: new structures stay generic
@@ -598,6 +617,12 @@ int jit_warm(int n, int dtype, const lq_gate* gates, int count) {
 int launch_jit(const JitKernel* k, void* psi, const TilePass& p, const TileOp* d_ops, uint64_t ninst,
                uint64_t inst_stride, uint64_t ops_stride, unsigned grid) {
     const int K = p.L + p.hb;
+    if (p.ntiles == 0 || (p.ntiles & (p.ntiles - 1)) != 0) {
+        // the kernels split a work index into instance and tile by shift
+        set_error("specialised tile pass: %llu tiles per state is not a power of two",
+                  (unsigned long long)p.ntiles);
+        return LQ_ERR_INVALID_ARGUMENT;
+    }
     TilePass pp = p;
     void* args[] = {&psi, &pp, &d_ops, &ninst, &inst_stride, &ops_stride};
     hipError_t e = hipModuleLaunchKernel(k->fn, grid, 1, 1, (unsigned)jit_threads(K), 1, 1, 0, stream(), args,

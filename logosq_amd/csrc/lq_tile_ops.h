@@ -584,8 +584,11 @@ __device__ __forceinline__ void j_group_first(V* tile, V* __restrict__ st, uint6
 // The first group split in two so that a tile's gather can be issued one
 // tile ahead (software pipelining across the tiles of a block):
 // j_first_fetch loads the subcube of the tile at `base` into r[16] without
-// waiting for it; j_first_apply applies the group to r and scatters it (into
-// the LDS tile after a barrier, or, LAST, straight back to the state).
+// waiting for it; j_first_apply takes r into registers, calls `next` (which
+// issues the next tile's j_first_fetch into r: those loads go out before this
+// tile's stores, so waiting for them later never waits for the stores),
+// applies the group and scatters it (into the LDS tile after a barrier, or,
+// LAST, straight back to the state).
 template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, typename V, typename GF>
 __device__ __forceinline__ void j_first_fetch(V* r, const V* __restrict__ st, uint64_t base, const uint64_t* hoff,
                                               const TilePass& p, GF&& gflip) {
@@ -609,10 +612,11 @@ __device__ __forceinline__ void j_first_fetch(V* r, const V* __restrict__ st, ui
     }
 }
 
-template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, bool LAST, typename T, typename V,
-          typename F, typename SF>
-__device__ __forceinline__ void j_first_apply(V* tile, V* __restrict__ st, const V* r, uint64_t base,
-                                              const uint64_t* hoff, const TilePass& p, F&& body, SF&& sflip) {
+template <int K, int HB, int NT, int S0, int S1, int S2, int S3, bool STREAM, bool LAST, bool EARLY, typename T,
+          typename V, typename F, typename SF, typename NX>
+__device__ __forceinline__ void j_first_apply(V* tile, V* __restrict__ st, V* r, uint64_t base,
+                                              const uint64_t* hoff, const TilePass& p, F&& body, SF&& sflip,
+                                              NX&& next) {
     constexpr uint32_t NSUB = (1u << K) >> 4;
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
@@ -628,8 +632,9 @@ __device__ __forceinline__ void j_first_apply(V* tile, V* __restrict__ st, const
             xr[c] = r[c].x;
             xi[c] = r[c].y;
         }
-        body(xr, xi, gb);
     }
+    if constexpr (EARLY) next();
+    if (act) body(xr, xi, gb);
     if constexpr (LAST) {
         if (act) {
             const uint64_t d0 = j_spread_bit<K, HB, S0>(p), d1 = j_spread_bit<K, HB, S1>(p);
@@ -662,6 +667,7 @@ __device__ __forceinline__ void j_first_apply(V* tile, V* __restrict__ st, const
         }
         __syncthreads();
     }
+    if constexpr (!EARLY) next();
 }
 
 // The last group of a tile, straight to the state: gathers from the LDS tile,
