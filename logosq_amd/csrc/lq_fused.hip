@@ -17,6 +17,7 @@
 // planning continues.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -352,6 +353,65 @@ static void build_groups(Plan& plan, TilePass& pass, int K) {
     }
 }
 
+// Order of a pass's ops for grouping: a list schedule that keeps every pair
+// of non-commuting ops in their order (two ops commute when each acts only
+// diagonally on the bits the other touches) and fills each register group
+// before opening the next: all ready ops whose non-diagonal bits lie in the
+// group's bits S are taken, then S grows by the ready op that adds the
+// fewest bits (the earliest on ties), up to kGroupBits bits.  Returns
+// positions into `take`.
+static const bool g_group_order = !(std::getenv("LQ_GROUP_ORDER") && std::getenv("LQ_GROUP_ORDER")[0] == '0');
+
+static std::vector<int> group_order(const std::vector<Prim>& prims, const std::vector<int>& take) {
+    const int m = (int)take.size();
+    std::vector<int> order;
+    order.reserve(m);
+    if (!g_group_order) {
+        for (int i = 0; i < m; ++i) order.push_back(i);
+        return order;
+    }
+    std::vector<std::vector<int>> succ(m);
+    std::vector<int> npred(m, 0);
+    for (int i = 0; i < m; ++i) {
+        const Prim& a = prims[take[i]];
+        for (int j = i + 1; j < m; ++j) {
+            const Prim& b = prims[take[j]];
+            if ((a.nd & (b.nd | b.ds)) || (b.nd & (a.nd | a.ds))) {
+                succ[i].push_back(j);
+                ++npred[j];
+            }
+        }
+    }
+    std::vector<char> done(m, 0);
+    auto put = [&](int i) {
+        done[i] = 1;
+        order.push_back(i);
+        for (int k : succ[i]) --npred[k];
+    };
+    while ((int)order.size() < m) {
+        uint64_t S = 0;
+        for (;;) {
+            // successors have larger indices: one sweep takes every chain that fits
+            for (int i = 0; i < m; ++i)
+                if (!done[i] && npred[i] == 0 && (prims[take[i]].nd & ~S) == 0) put(i);
+            int best = -1, best_add = 0;
+            for (int i = 0; i < m; ++i) {
+                if (done[i] || npred[i] != 0) continue;
+                const uint64_t nd = prims[take[i]].nd;
+                if (__builtin_popcountll(S | nd) > kGroupBits) continue;
+                const int add = __builtin_popcountll(nd & ~S);
+                if (best < 0 || add < best_add) {
+                    best = i;
+                    best_add = add;
+                }
+            }
+            if (best < 0) break;
+            S |= prims[take[best]].nd;
+        }
+    }
+    return order;
+}
+
 void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan) {
     plan.n = n;
     plan.steps.clear();
@@ -407,7 +467,8 @@ void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan
             st.pass.op_first = (int)plan.ops.size();
             st.pass.op_count = (int)take.size();
             st.pass.ntiles = 1ull << (n - L - st.pass.hb);
-            for (int idx : take) {
+            for (int pos : group_order(prims, take)) {
+                const int idx = take[pos];
                 plan.ops.push_back(to_tile_op(prims[idx], L, st.pass.hpos, st.pass.hb));
                 plan.op_src.push_back(prims[idx].src);
             }
