@@ -285,13 +285,17 @@ __device__ __forceinline__ void j_diag(T* xr, T* xi, const TileOp& op, int tb0, 
 // ts (1 where the op's in-subcube controls CINV are not all set), and the
 // run applied from its 16-entry table (entries agree on the bits outside
 // DEP, the subcube positions the run depends on).
+// (The phases are read as block-uniform values and selected in registers, as
+// in j_uphase_mul.)
 template <int S0, int S1, int CINV>
 __device__ __forceinline__ void j_phase_mul(double& pr, double& pi, int ts, const TileOp& op) {
+    const double r0 = op.m[0], i0 = op.m[1], r1 = op.m[2], i1 = op.m[3];
+    const double r2 = op.m[4], i2 = op.m[5], r3 = op.m[6], i3 = op.m[7];
     if ((ts & CINV) != CINV) return;
-    const int b0 = S0 >= 0 ? ((ts >> (S0 >= 0 ? S0 : 0)) & 1) : 0;
-    const int b1 = S1 >= 0 ? ((ts >> (S1 >= 0 ? S1 : 0)) & 1) : 0;
-    const int k = b0 | (b1 << 1);
-    const double dr = op.m[2 * k], di = op.m[2 * k + 1];
+    const bool b0 = S0 >= 0 && ((ts >> (S0 >= 0 ? S0 : 0)) & 1);
+    const bool b1 = S1 >= 0 && ((ts >> (S1 >= 0 ? S1 : 0)) & 1);
+    const double dr = b1 ? (b0 ? r3 : r2) : (b0 ? r1 : r0);
+    const double di = b1 ? (b0 ? i3 : i2) : (b0 ? i1 : i0);
     const double nr = pr * dr - pi * di;
     pi = pr * di + pi * dr;
     pr = nr;
