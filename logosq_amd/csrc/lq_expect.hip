@@ -442,6 +442,8 @@ int orbit_max_groups(int dtype) {
     return kOrbitMax;
 }
 
+// An item's terms are listed generator by generator (0 .. m-1), then the
+// diagonal terms: the orbit kernel walks them in that order.
 void plan_orbits(const PauliSum& ps, const std::vector<int>& groups, int diag, int maxm, std::vector<OrbitSet>& out) {
     out.clear();
     maxm = std::max(1, std::min(maxm, kOrbitMax));

@@ -51,22 +51,6 @@ struct OrbitTerm {
 };
 
 #ifdef __HIPCC__
-template <int J, int M, typename V>
-__device__ __forceinline__ double orbit_group_sum(const V* a, uint32_t sigma, int imag) {
-    double wr = 0.0, wi = 0.0;
-#pragma unroll
-    for (int c = 0; c < (1 << M); ++c) {
-        if (c & (1 << J)) continue;
-        const V u = a[c], v = a[c | (1 << J)];
-        const double zr = (double)u.x * (double)v.x + (double)u.y * (double)v.y;
-        const double zi = (double)u.x * (double)v.y - (double)u.y * (double)v.x;
-        const bool neg = (sigma >> c) & 1u;
-        wr += neg ? -zr : zr;
-        wi += neg ? -zi : zi;
-    }
-    return imag ? wi : wr;
-}
-
 // Sum over the orbits k0, k0 + stride, ... < it.work of the item's terms.
 template <int M, typename V>
 __device__ __forceinline__ double orbit_acc(const V* __restrict__ s, const OrbitItem& it,
@@ -92,21 +76,51 @@ __device__ __forceinline__ double orbit_acc(const V* __restrict__ s, const Orbit
         V a[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) a[c] = s[i0 ^ xc[c]];
-        for (int t = 0; t < it.term_count; ++t) {
-            const OrbitTerm q = tm[it.term_first + t];
-            const double g = (__popcll(i0 & q.zy) & 1) ? -q.g : q.g;
-            double w = 0.0;
-            if (q.gen < 0) {
+        // terms come ordered by group (plan_orbits: generators 0 .. M-1,
+        // then the diagonal terms): each group's pair products are formed
+        // once and kept only while its terms are summed
+        int t = it.term_first;
+        const int t1 = it.term_first + it.term_count;
+#pragma unroll
+        for (int J = 0; J < M; ++J) {
+            if (t < t1 && tm[t].gen == J) {
+                double zr[C / 2 > 0 ? C / 2 : 1], zi[C / 2 > 0 ? C / 2 : 1];
+                int np = 0;
 #pragma unroll
                 for (int c = 0; c < C; ++c) {
-                    const double p = (double)a[c].x * (double)a[c].x + (double)a[c].y * (double)a[c].y;
-                    w += ((q.sigma >> c) & 1u) ? -p : p;
+                    if (c & (1 << J)) continue;
+                    const V u = a[c], v = a[c | (1 << J)];
+                    zr[np] = (double)u.x * (double)v.x + (double)u.y * (double)v.y;
+                    zi[np] = (double)u.x * (double)v.y - (double)u.y * (double)v.x;
+                    ++np;
                 }
-            } else {
-                if constexpr (M >= 1) if (q.gen == 0) w = orbit_group_sum<0, M>(a, q.sigma, q.imag);
-                if constexpr (M >= 2) if (q.gen == 1) w = orbit_group_sum<1, M>(a, q.sigma, q.imag);
-                if constexpr (M >= 3) if (q.gen == 2) w = orbit_group_sum<2, M>(a, q.sigma, q.imag);
-                if constexpr (M >= 4) if (q.gen == 3) w = orbit_group_sum<3, M>(a, q.sigma, q.imag);
+                do {
+                    const OrbitTerm q = tm[t];
+                    const double g = (__popcll(i0 & q.zy) & 1) ? -q.g : q.g;
+                    double wr = 0.0, wi = 0.0;
+                    int e = 0;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        if (c & (1 << J)) continue;
+                        const bool neg = (q.sigma >> c) & 1u;
+                        wr += neg ? -zr[e] : zr[e];
+                        wi += neg ? -zi[e] : zi[e];
+                        ++e;
+                    }
+                    acc += g * (q.imag ? wi : wr);
+                    ++t;
+                } while (t < t1 && tm[t].gen == J);
+            }
+        }
+        // then the diagonal terms
+        for (; t < t1; ++t) {
+            const OrbitTerm q = tm[t];
+            const double g = (__popcll(i0 & q.zy) & 1) ? -q.g : q.g;
+            double w = 0.0;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const double p = (double)a[c].x * (double)a[c].x + (double)a[c].y * (double)a[c].y;
+                w += ((q.sigma >> c) & 1u) ? -p : p;
             }
             acc += g * w;
         }
