@@ -412,6 +412,10 @@ static std::vector<int> group_order(const std::vector<Prim>& prims, const std::v
     return order;
 }
 
+// Pass high bits chosen among windows of consecutive bits as well as first
+// fit (LQ_PLAN_SEARCH=0: first fit only).
+static const bool g_plan_search = !(std::getenv("LQ_PLAN_SEARCH") && std::getenv("LQ_PLAN_SEARCH")[0] == '0');
+
 void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan) {
     plan.n = n;
     plan.steps.clear();
@@ -457,6 +461,46 @@ void plan_passes(int n, const std::vector<Prim>& prims, int K, int L, Plan& plan
                 if (need && __builtin_popcountll(H | need) <= HB) H |= need;
             }
             for (int b = L; b < n && __builtin_popcountll(H) < HB; ++b) H |= 1ull << b;
+            if (g_plan_search && HB > 0 && n > K) {
+                // a fixed high set takes every gate that fits it and is not
+                // held back by a deferred one: keep the window of HB
+                // consecutive high bits that takes the most, when it beats
+                // the first-fit set (the first-fit set also takes gates
+                // that its later-added bits would allow)
+                auto take_with = [&](uint64_t Hs, std::vector<int>* t, std::vector<int>* k) {
+                    uint64_t bn = 0, bs = 0;
+                    size_t count = 0;
+                    for (int idx : pending) {
+                        const Prim& p = prims[idx];
+                        const bool blocked = (p.nd & (bn | bs)) || (p.ds & bn);
+                        if (!blocked && ((p.nd & ~lowmask) & ~Hs) == 0) {
+                            ++count;
+                            if (t) t->push_back(idx);
+                        } else {
+                            bn |= p.nd;
+                            bs |= p.ds;
+                            if (k) k->push_back(idx);
+                        }
+                    }
+                    return count;
+                };
+                size_t best = take_with(H, nullptr, nullptr);
+                uint64_t bestH = H;
+                for (int b = L; b + HB <= n; ++b) {
+                    const uint64_t Hw = ((HB >= 64 ? ~0ull : ((1ull << HB) - 1ull)) << b);
+                    const size_t c = take_with(Hw, nullptr, nullptr);
+                    if (c > best) {
+                        best = c;
+                        bestH = Hw;
+                    }
+                }
+                if (bestH != H || take.size() != best) {
+                    H = bestH;
+                    take.clear();
+                    keep.clear();
+                    take_with(H, &take, &keep);
+                }
+            }
             Step st;
             st.normalize = false;
             std::memset(&st.pass, 0, sizeof(st.pass));
