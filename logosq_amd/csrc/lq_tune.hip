@@ -131,24 +131,28 @@ extern "C" int lq_autotune(int num_qubits, int dtype, char* report, size_t repor
         snprintf(buf, sizeof(buf), "], \"unit_order_choice\": %d, \"tile_variants\": [", best_order);
         rep += buf;
     }
-    // ---- fused tiles: a depth-4 brickwork of RX/RY/RZ + CNOT layers
+    // ---- fused tiles: a depth-20 brickwork of RX/RY/RZ + CNOT layers (the
+    // planner's windows run several layers per pass, so a shallow sample
+    // ranks the geometries differently from deep circuits: depth 4 and 12
+    // chose 2^13 / 4 bits, 137 ms on the 30-qubit depth-20 brickwork against
+    // 131 ms at 2^12 / 4)
     std::vector<lq_gate> circ;
-    for (int layer = 0; layer < 4; ++layer) {
+    for (int layer = 0; layer < 20; ++layer) {
         for (int q = 0; q < n; ++q)
             circ.push_back(mk_gate(LQ_GATE_RX + ((q + layer) % 3), q, -1, 0.1 * (q + 1) + layer));
         for (int q = layer % 2; q + 1 < n; q += 2) circ.push_back(mk_gate(LQ_GATE_CNOT, q, q + 1, 0));
     }
     int t0c, t0s, l0;
     get_fused_config(&t0c, &t0s, &l0);
-    const int tiles128[4] = {10, 11, 12, 13};
-    const int tiles64[4] = {11, 12, 13, 13};
-    const int lows[3] = {4, 5, 6};
+    // (2^10 tiles and 6 low bits never won at 20-30 qubits: not timed)
+    const int tiles128[3] = {11, 12, 13};
+    const int tiles64[3] = {11, 12, 13};
+    const int lows[2] = {4, 5};
     float bestf = 1e30f;
     int bt = dtype == LQ_C128 ? t0c : t0s, bl = l0;
     first = true;
-    for (int ti = 0; ti < 4 && !rc; ++ti) {
-        if (dtype == LQ_C64 && ti == 3) break;
-        for (int li = 0; li < 3 && !rc; ++li) {
+    for (int ti = 0; ti < 3 && !rc; ++ti) {
+        for (int li = 0; li < 2 && !rc; ++li) {
             int tb = dtype == LQ_C128 ? tiles128[ti] : tiles64[ti];
             if (tb > n) continue;
             if (dtype == LQ_C128) rc = set_fused_config_internal(tb, t0s, lows[li]);
