@@ -77,6 +77,25 @@ def algorithmic_bytes(descs, n, c):
     return total
 
 
+FP64_PEAK_GFLOPS = 78600.0  # MI355X dense FP64 vector (FMA counted as two flops)
+
+
+def algorithmic_flops(descs, n):
+    """Floating-point operations of a circuit's gates at their minimum
+    count per amplitude: a real-coefficient rotation (RX, RY: a multiply and
+    a multiply-add per output value) 6, a phase gate applied to every
+    amplitude (RZ: one complex multiply) 6, a CNOT none (a permutation);
+    other kinds are not counted (None)."""
+    from logosq_amd import abi
+    per = {abi.RX: 6, abi.RY: 6, abi.RZ: 6, abi.CNOT: 0}
+    total = 0
+    for g in descs:
+        if g.kind not in per:
+            return None
+        total += per[g.kind] * (1 << n)
+    return total
+
+
 def line_bytes(g, n, c, line=128):
     """HBM bytes a gate moves when every 128-byte line holding an amplitude
     it changes is read and written whole.  A fixed bit (a control, the
@@ -718,9 +737,17 @@ def main():
              "effective_gate_GBps": round(bytes_gates / fused_s / 1e9, 1),
              "pass_kernel_GBps": round(2 * c * N / pass_s / 1e9, 1),
              "pass_kernel_hbm_frac": round(2 * c * N / pass_s / 1e9 / HBM_PEAK_GBS, 4),
+             # the circuit's two floors: its passes' reads and writes at the
+             # measured copy rate, and its gates' FP64 work at the vector peak
+             "hbm_floor_ms": round(passes * 2 * c * N / copy_gbs / 1e6, 2),
              "pass_kernel": "lq_pass (specialised)" if jit_launches else "k_tileg (generic)",
              "specialised_kernels": jit_kernels, "specialised_compile_s": round(jit_compile_s, 2),
              "specialised_launches": jit_launches}
+    flops = algorithmic_flops(descs, a.qubits) if dtype == lq.C128 else None
+    if flops is not None:
+        fp64_floor_ms = flops / (FP64_PEAK_GFLOPS * 1e9) * 1e3
+        fused["fp64_floor_ms"] = round(fp64_floor_ms, 2)
+        fused["frac_of_floor"] = round(max(fused["hbm_floor_ms"], fp64_floor_ms) / (fused_s * 1e3), 4)
     st = None
     c64 = None
     if dtype == lq.C128 and not a.no_extra:
