@@ -747,6 +747,10 @@ def main():
     if flops is not None:
         fp64_floor_ms = flops / (FP64_PEAK_GFLOPS * 1e9) * 1e3
         fused["fp64_floor_ms"] = round(fp64_floor_ms, 2)
+        # the same gates in VALU issue slots: each multiply takes a slot of
+        # the FMA rate, so a rotation is 4 slots (8 flop-equivalents) per
+        # amplitude
+        fused["fp64_issue_floor_ms"] = round(fp64_floor_ms * 8 / 6, 2)
         fused["frac_of_floor"] = round(max(fused["hbm_floor_ms"], fp64_floor_ms) / (fused_s * 1e3), 4)
     st = None
     c64 = None
