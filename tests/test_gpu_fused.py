@@ -97,3 +97,32 @@ def test_fused_qft_12(hip):
     lq.algorithms.qft_circuit(n).execute(st, abi.EXEC_FUSED)
     ref = oracle.execute(oracle.normalize(psi0)[0], n, oracle.qft_gates(n))
     assert np.max(np.abs(st.vector() - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("tile,low", [(12, 4), (12, 5), (13, 4)])
+def test_fused_autotuner_geometries(hip, tile_config, tile, low):
+    """The geometries the autotuner chooses between at 30 qubits, on a
+    17-qubit state (a pass's high bits picked among several windows): a
+    depth-12 brickwork on the run-time specialised pass kernels and a random
+    circuit on the generic tile kernel, complex128 <= 1e-12 and complex64
+    <= 2e-6 against the oracle."""
+    from logosq_amd import jit
+    from tests import modes
+    say("fused autotuner geometry tile=%d low=%d" % (tile, low))
+    tile_config(tile, tile, low)
+    n = 17
+    rng = np.random.default_rng(7000 + 10 * tile + low)
+    brick = lq.algorithms.brickwork_circuit(n, 12, tile + low).descriptors()
+    rand = random_circuit(n, 120, rng, with_dense=False)
+    try:
+        for gates, mode in ((brick, jit.ALWAYS), (rand, jit.OFF)):
+            jit.set_jit(mode)
+            psi = bf.random_state(n, rng)
+            want = oracle.execute(psi, n, gates)
+            launches = jit.jit_stats()["launches"]
+            assert np.max(np.abs(run(psi, n, gates, lq.C128, abi.EXEC_FUSED) - want)) <= 1e-12
+            assert np.max(np.abs(run(psi, n, gates, lq.C64, abi.EXEC_FUSED) - want)) <= 2e-6
+            ran = jit.jit_stats()["launches"] - launches
+            assert (ran > 0) == (mode == jit.ALWAYS), (mode, ran)
+    finally:
+        modes.restore_defaults()

@@ -84,3 +84,24 @@ def test_small_state_single_pass():
     circ = lq.algorithms.brickwork_circuit(10, 20, 5)
     order, pidx, npass = plan(10, circ.descriptors())
     assert npass == 1
+
+
+@pytest.mark.parametrize("tile,low", [(12, 4), (12, 5), (13, 4)])
+@pytest.mark.parametrize("n,seed", [(14, 1), (15, 2), (16, 3)])
+def test_window_search_geometries(n, seed, tile, low):
+    """The tile geometries the autotuner chooses between at 30 qubits, on
+    states a few bits wider than a tile (so a pass's high bits are picked
+    among several windows): a brickwork circuit and a random circuit, each
+    planned order replayed by the oracle against the in-order result, every
+    gate once, passes non-decreasing."""
+    rng = np.random.default_rng(1000 * n + seed)
+    brick = lq.algorithms.brickwork_circuit(n, 12, seed).descriptors()
+    for gates in (brick, random_circuit(n, 160, rng, with_dense=False)):
+        order, pidx, npass = plan(n, gates, tile, low)
+        assert sorted(order) == list(range(len(gates)))
+        assert all(pidx[i] <= pidx[i + 1] for i in range(len(pidx) - 1))
+        assert 1 <= npass <= len(gates)
+        psi = bf.random_state(n, rng)
+        want = oracle.execute(psi, n, gates)
+        got = oracle.execute(psi, n, [gates[i] for i in order])
+        assert np.max(np.abs(got - want)) < 1e-12
